@@ -1,0 +1,1213 @@
+// Solver engine and C ABI of libfvm_hip.so.
+//
+// A batch holds one or more flow cases on one or more meshes.  Host set-up
+// (once): reference topology and FP64 geometry per mesh, tiled renumbered
+// layout, freestream initial state (Initialization, gpu_init.cu:63-93), one
+// device arena.  Run: hipGraph-captured sequences of time steps; each step is
+// four stages of (flux_tile, update_tile) over all tiles of all cases, then
+// finalize_step; the host polls the per-case control block between graphs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fvm_internal.h"
+#include "fvm_launch.h"
+#include "fvm_plain.h"
+
+namespace fvm {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+#define HIP_OK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      throw Error{FVM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)};     \
+  } while (0)
+
+static const double kR = 287.06;              // gpu_fvm.cu:8
+static const double kPI = 3.14159265358979;   // gpu_fvm.cu:9
+static const double kRK[4] = {0.0833, 0.2069, 0.4265, 1.0};  // gpu_fvm.cu:10
+
+// ---------------------------------------------------------------------------
+// finalize_step: per case residual and stop test
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void neumaier(double& s, double& c, double v) {
+  const double t = s + v;
+  if (fabs(s) >= fabs(v))
+    c += (s - t) + v;
+  else
+    c += (v - t) + s;
+  s = t;
+}
+
+__global__ void __launch_bounds__(64) finalize_kernel(CaseCtl* ctl, const double* partial,
+                                                      double* hist, int hist_stride, int sequential) {
+  CaseCtl& c = ctl[blockIdx.x];
+  if (!c.active) return;
+  __shared__ double part[64];
+  double drou;
+  if (sequential) {
+    // the reference's host loop over block partials, gpu_solver.cu:2410-2414
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int k = 0; k < c.ntile; k++) s += partial[c.tile0 + k];
+      part[0] = s;
+    }
+    __syncthreads();
+    drou = part[0];
+  } else {
+    const int n = c.ntile, per = (n + 63) / 64;
+    const int b = threadIdx.x * per, e = min(n, b + per);
+    double s = 0.0, cc = 0.0;
+    for (int k = b; k < e; k++) neumaier(s, cc, partial[c.tile0 + k]);
+    part[threadIdx.x] = s + cc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s2 = 0.0, c2 = 0.0;
+      for (int k = 0; k < 64; k++) neumaier(s2, c2, part[k]);
+      part[0] = s2 + c2;
+    }
+    __syncthreads();
+    drou = part[0];
+  }
+  if (threadIdx.x == 0) {
+    const int step = c.step + 1;
+    c.step = step;
+    if (step == 1) c.epsm = drou;
+    const double resd = sqrt((drou + 1.0E-32) / c.epsm);
+    c.resd = resd;
+    if (step < hist_stride) hist[(size_t)blockIdx.x * hist_stride + step] = resd;
+    if (!isfinite(resd)) c.status = FVM_ERR_NAN;
+    if (!(step < c.max_steps && resd > c.epsl)) c.active = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forces: first-order Roe wall flux of the current state, FP64, wall-id order
+// ---------------------------------------------------------------------------
+struct WallTable {
+  const int* cell;       // state index of the wall edge's cell (case-independent part)
+  const double* vx;
+  const double* vy;
+  const double* vn;
+};
+
+template <typename Load>
+__device__ void forces_body(Load load, const WallTable w, int w0, int nw, const FlowT<double>& f,
+                            double* cl, double* cd) {
+  double fx = 0, cfx = 0, fy = 0, cfy = 0, sx = 0, csx = 0, sy = 0, csy = 0;
+  for (int k = w0; k < w0 + nw; k++) {
+    const Vec4<double> L = load(w.cell[k]);
+    const double vx = w.vx[k], vy = w.vy[k], vn = w.vn[k];
+    const Vec4<double> G = wall_ghost(L, vx, vy, vn);
+    const Vec4<double> F = roe_flux(L, G, vy / vn, vx / vn, f.gamma);
+    neumaier(fx, cfx, F.du * vn);
+    neumaier(fy, cfy, F.dv * vn);
+    neumaier(sx, csx, vy);
+    neumaier(sy, csy, vx);
+  }
+  const double Fx = (fx + cfx) - f.pis * (sx + csx);
+  const double Fy = (fy + cfy) - f.pis * (sy + csy);
+  const double q = 0.5;
+  *cl = (-Fx * f.vi + Fy * f.ui) / q;
+  *cd = (Fx * f.ui + Fy * f.vi) / q;
+}
+
+template <typename R>
+__global__ void forces_tiled(const SVec4<R>* W, const int* st0, const int* geo_w0, const int* nwall,
+                             WallTable w, const FlowT<double>* flow, double* cl, double* cd) {
+  const int c = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  const SVec4<R>* Wc = W + st0[c];
+  auto load = [&](int i) {
+    const SVec4<R> s = Wc[i];
+    return Vec4<double>{(double)s.d, (double)s.du, (double)s.dv, (double)s.de};
+  };
+  forces_body(load, w, geo_w0[c], nwall[c], flow[c], cl + c, cd + c);
+}
+
+__global__ void forces_plain(PlainDev p, WallTable w, int nw, const FlowT<double>* flow, double* cl,
+                             double* cd) {
+  if (threadIdx.x != 0) return;
+  auto load = [&](int i) { return Vec4<double>{p.w[0][i], p.w[1][i], p.w[2][i], p.w[3][i]}; };
+  forces_body(load, w, 0, nw, *flow, cl, cd);
+}
+
+// ---------------------------------------------------------------------------
+// Device arena: one allocation, carved in 256-byte aligned pieces
+// ---------------------------------------------------------------------------
+class Arena {
+ public:
+  // The same sequence of take() calls runs twice: first to size the arena
+  // (no memory yet, take() returns nullptr), then after allocate() to carve it.
+  ~Arena() {
+    if (base_) (void)hipFree(base_);
+  }
+  void allocate() {
+    HIP_OK(hipMalloc(&base_, std::max<size_t>(off_, 256)));
+    size_ = std::max<size_t>(off_, 256);
+    off_ = 0;
+  }
+  template <typename T>
+  T* take(size_t n) {
+    const size_t at = off_;
+    off_ += align(n * sizeof(T));
+    if (!base_) return nullptr;
+    if (off_ > size_) throw Error{FVM_ERR_ALLOC, "internal: arena overflow"};
+    return reinterpret_cast<T*>(static_cast<char*>(base_) + at);
+  }
+  size_t bytes() const { return size_; }
+
+ private:
+  static size_t align(size_t b) { return (b + 255) & ~size_t(255); }
+  void* base_ = nullptr;
+  size_t size_ = 0, off_ = 0;
+};
+
+template <typename T>
+static void h2d(T* dst, const std::vector<T>& src, hipStream_t s) {
+  if (!src.empty()) HIP_OK(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// ---------------------------------------------------------------------------
+// Batch
+// ---------------------------------------------------------------------------
+struct CaseInfo {
+  int mesh;
+  fvm_flow_params flow;
+  FlowT<double> f64;
+  double rhoE_inf;
+};
+
+class Batch {
+ public:
+  Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_mesh,
+        const fvm_flow_params* flows, const fvm_scheme_opts& o);
+  ~Batch();
+  void run(int max_steps, double epsl, double* hist, int* steps_done, int* case_status);
+  void get_state(int icase, double* rho, double* rhou, double* rhov, double* rhoe);
+  void set_state(int icase, const double* rho, const double* rhou, const double* rhov,
+                 const double* rhoe);
+  void forces(double* cl, double* cd);
+  void kernel_times(int reps, double ms[2]);
+  double last_ms = 0.0;
+  int64_t last_csu = 0;
+  int ncase() const { return (int)cases_.size(); }
+  int ncell(int icase) const { return topo_[cases_[icase].mesh].ncell; }
+  void config(int32_t cfg[4]) const {
+    cfg[0] = tile_cells_;
+    cfg[1] = threads_;
+    cfg[2] = steps_per_graph_;
+    cfg[3] = total_tiles_;
+  }
+
+ private:
+  template <typename R>
+  void setup_tiled();
+  void setup_plain();
+  template <typename R>
+  TiledArgs<R> tiled_args();
+  template <typename R>
+  void enqueue_tiled_steps(int nsteps);
+  template <typename R>
+  void time_tiled(int reps, double ms[2]);
+  void enqueue_plain_steps(int nsteps);
+  void enqueue_steps(int nsteps);
+  void build_graph();
+
+  fvm_scheme_opts o_;
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  hipGraphExec_t graph_ = nullptr;
+  std::vector<Topology> topo_;
+  std::vector<TiledMesh> tiled_;
+  std::vector<CaseInfo> cases_;
+  Arena arena_;
+  int tile_cells_ = 0, threads_ = 256, steps_per_graph_ = 20, total_tiles_ = 0;
+  size_t lds_a_ = 0, lds_b_ = 0;
+  // control
+  CaseCtl* d_ctl_ = nullptr;
+  std::vector<CaseCtl> h_ctl_;
+  double* d_partial_ = nullptr;
+  double* d_hist_ = nullptr;
+  int hist_stride_ = 1;
+  std::vector<int> state_off_;   // per case: offset in the per-cell state arrays (tiled)
+  // tiled buffers (type-erased; R = float or double)
+  void* d_flow_ = nullptr;
+  FlowT<double>* d_flow64_ = nullptr;
+  void *d_cg_ = nullptr, *d_edges_ = nullptr;
+  int4 *d_nbr_ = nullptr, *d_slot_ = nullptr, *d_nloc_ = nullptr;
+  int* d_halo_ = nullptr;
+  TileDesc* d_tiles_ = nullptr;
+  void *d_w0_ = nullptr, *d_wc_ = nullptr, *d_res_ = nullptr, *d_dt_ = nullptr;
+  void *d_tile_min_ = nullptr, *d_dtg_ = nullptr;
+  // forces tables (tiled)
+  WallTable walls_{};
+  int *d_case_st0_ = nullptr, *d_case_w0_ = nullptr, *d_case_nw_ = nullptr;
+  double *d_cl_ = nullptr, *d_cd_ = nullptr;
+  // plain
+  std::vector<PlainDev> plain_;
+  std::vector<WallTable> plain_walls_;
+};
+
+static FlowT<double> make_flow(const fvm_flow_params& p, double eps, double* rhoE_inf) {
+  // Initialization, gpu_init.cu:63-93 (non-dimensional freestream) and
+  // CalculationPTA gpu_init.cu:49-61 for PIS = w[1].P.
+  FlowT<double> f{};
+  f.gamma = p.gamma;
+  f.R = kR;
+  const double alpha = p.alpha_deg * kPI / 180.0;
+  f.ain = std::sqrt(p.gamma * kR * p.t_in);
+  f.vin = p.mach * f.ain;
+  f.tin = p.t_in;
+  f.ui = std::cos(alpha);
+  f.vi = std::sin(alpha);
+  const double rho = 1.0, rhou = std::cos(alpha), rhov = std::sin(alpha);
+  const double rhoE = kR * p.t_in / (f.vin * f.vin) / (p.gamma - 1.0) + 0.50;
+  *rhoE_inf = rhoE;
+  const double U = rhou / rho, V = rhov / rho;
+  const double E = rhoE / rho - 0.5 * (U * U + V * V);
+  f.pis = (p.gamma - 1.0) * rho * E;
+  f.cfl = p.cfl;
+  f.eps = eps;
+  return f;
+}
+
+template <typename R>
+static FlowT<R> cast_flow(const FlowT<double>& f) {
+  return FlowT<R>{(R)f.gamma, (R)f.R, (R)f.ui, (R)f.vi, (R)f.ain, (R)f.tin, (R)f.vin, (R)f.pis,
+                  (R)f.cfl, (R)f.eps};
+}
+
+Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_mesh,
+             const fvm_flow_params* flows, const fvm_scheme_opts& o)
+    : o_(o) {
+  if (nmesh < 1 || ncase < 1 || !meshes || !case_mesh || !flows)
+    throw Error{FVM_ERR_ARGUMENT, "batch: need nmesh >= 1, ncase >= 1 and non-NULL arrays"};
+  if (o.order != 1 && o.order != 2) throw Error{FVM_ERR_ARGUMENT, "opts.order must be 1 or 2"};
+  if (o.precision != FVM_FP64 && o.precision != FVM_FP32)
+    throw Error{FVM_ERR_ARGUMENT, "opts.precision must be FVM_FP64 or FVM_FP32"};
+  if (o.kernels != FVM_KERNELS_TILED && o.kernels != FVM_KERNELS_PLAIN)
+    throw Error{FVM_ERR_ARGUMENT, "opts.kernels must be FVM_KERNELS_TILED or FVM_KERNELS_PLAIN"};
+  if (o.kernels == FVM_KERNELS_PLAIN && o.precision != FVM_FP64)
+    throw Error{FVM_ERR_ARGUMENT, "the plain kernels run in FP64 only"};
+  if (o.threads != 0 && (o.threads < 64 || o.threads > 256 || o.threads % 64 != 0))
+    throw Error{FVM_ERR_ARGUMENT, "opts.threads must be 0 or one of 64, 128, 192, 256"};
+  for (int i = 0; i < ncase; i++) {
+    if (case_mesh[i] < 0 || case_mesh[i] >= nmesh)
+      throw Error{FVM_ERR_ARGUMENT, "case_mesh[" + std::to_string(i) + "] out of range"};
+    const fvm_flow_params& p = flows[i];
+    if (!(p.gamma > 1.0) || !(p.mach > 0.0) || !(p.t_in > 0.0) || !(p.cfl > 0.0))
+      throw Error{FVM_ERR_ARGUMENT, "flow[" + std::to_string(i) + "]: need gamma > 1, mach > 0, t_in > 0, cfl > 0"};
+  }
+  // host pre-processing first: it validates the meshes and the tile tables
+  for (int m = 0; m < nmesh; m++) topo_.push_back(build_topology(meshes[m]));
+  for (int i = 0; i < ncase; i++) {
+    CaseInfo ci;
+    ci.mesh = case_mesh[i];
+    ci.flow = flows[i];
+    ci.f64 = make_flow(flows[i], o.eps_smooth, &ci.rhoE_inf);
+    cases_.push_back(ci);
+  }
+  if (o.kernels == FVM_KERNELS_TILED) {
+    tile_cells_ = o.tile_cells > 0 ? o.tile_cells : (o.precision == FVM_FP64 ? 160 : 256);
+    threads_ = o.threads > 0 ? o.threads : 256;
+    for (int m = 0; m < nmesh; m++) tiled_.push_back(build_tiled(topo_[m], tile_cells_));
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    throw Error{FVM_ERR_NO_DEVICE, "no HIP device available"};
+  if (o.device < 0 || o.device >= ndev)
+    throw Error{FVM_ERR_NO_DEVICE, "device ordinal " + std::to_string(o.device) + " out of range"};
+  device_ = o.device;
+  HIP_OK(hipSetDevice(device_));
+  HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  HIP_OK(hipEventCreate(&ev0_));
+  HIP_OK(hipEventCreate(&ev1_));
+  if (o.kernels == FVM_KERNELS_PLAIN)
+    setup_plain();
+  else if (o.precision == FVM_FP64)
+    setup_tiled<double>();
+  else
+    setup_tiled<float>();
+  steps_per_graph_ = o.steps_per_graph > 0 ? o.steps_per_graph : 20;
+  HIP_OK(hipStreamSynchronize(stream_));
+}
+
+Batch::~Batch() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (graph_) (void)hipGraphExecDestroy(graph_);
+  if (d_hist_) (void)hipFree(d_hist_);
+  if (ev0_) (void)hipEventDestroy(ev0_);
+  if (ev1_) (void)hipEventDestroy(ev1_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+template <typename R>
+void Batch::setup_tiled() {
+  const int ncase = (int)cases_.size(), nmesh = (int)topo_.size();
+  int ML = 0, ME = 0;
+  for (auto& t : tiled_) {
+    ML = std::max(ML, t.max_local);
+    ME = std::max(ME, t.max_edges);
+  }
+  lds_a_ = flux_tile_lds<R>(ML, ME);
+  lds_b_ = update_tile_lds<R>(ML);
+  if (lds_a_ > 160 * 1024)
+    throw Error{FVM_ERR_ARGUMENT, "tile_cells too large: flux kernel needs " +
+                                      std::to_string(lds_a_) + " bytes of LDS"};
+  HIP_OK(set_tile_lds_limit<R>(std::max(lds_a_, lds_b_)));
+  // geometry offsets per mesh
+  std::vector<int> geo0(nmesh), halo0(nmesh), edge0(nmesh), wall0(nmesh);
+  int ncell_all = 0, nhalo_all = 0, nedge_all = 0, nwall_all = 0;
+  for (int m = 0; m < nmesh; m++) {
+    geo0[m] = ncell_all;
+    halo0[m] = nhalo_all;
+    edge0[m] = nedge_all;
+    wall0[m] = nwall_all;
+    ncell_all += tiled_[m].ncell;
+    nhalo_all += (int)tiled_[m].halo.size();
+    nedge_all += (int)tiled_[m].e_l.size();
+    nwall_all += (int)tiled_[m].w_cell.size();
+  }
+  // case state offsets and the batch tile list
+  std::vector<TileDesc> tiles;
+  state_off_.assign(ncase, 0);
+  h_ctl_.assign(ncase, CaseCtl{});
+  int ncell_state = 0;
+  for (int c = 0; c < ncase; c++) {
+    const int m = cases_[c].mesh;
+    const TiledMesh& T = tiled_[m];
+    state_off_[c] = ncell_state;
+    h_ctl_[c].tile0 = (int)tiles.size();
+    h_ctl_[c].ntile = T.ntile;
+    h_ctl_[c].resd = 100.0;
+    h_ctl_[c].status = FVM_OK;
+    for (int k = 0; k < T.ntile; k++) {
+      TileDesc d;
+      d.icase = c;
+      d.geo0 = geo0[m];
+      d.st0 = ncell_state;
+      d.cell0 = T.tile_cell0[k];
+      d.ncell = T.tile_cell0[k + 1] - T.tile_cell0[k];
+      d.halo0 = halo0[m] + T.tile_halo0[k];
+      d.nhalo = T.tile_halo0[k + 1] - T.tile_halo0[k];
+      d.edge0 = edge0[m] + T.tile_edge0[k];
+      d.nedge = T.tile_edge0[k + 1] - T.tile_edge0[k];
+      tiles.push_back(d);
+    }
+    ncell_state += T.ncell;
+  }
+  total_tiles_ = (int)tiles.size();
+  FlowT<R>* flow = nullptr;
+  CellGeom<R>* cg = nullptr;
+  EdgeRec<R>* edges = nullptr;
+  SVec4<R>* w0 = nullptr;
+  int* wcell = nullptr;
+  double *wvx = nullptr, *wvy = nullptr, *wvn = nullptr;
+  auto carve = [&]() {
+    d_ctl_ = arena_.take<CaseCtl>(ncase);
+    flow = arena_.take<FlowT<R>>(ncase);
+    d_flow_ = flow;
+    d_flow64_ = arena_.take<FlowT<double>>(ncase);
+    cg = arena_.take<CellGeom<R>>(ncell_all);
+    d_cg_ = cg;
+    d_nbr_ = arena_.take<int4>(ncell_all);
+    d_slot_ = arena_.take<int4>(ncell_all);
+    d_nloc_ = arena_.take<int4>(ncell_all);
+    d_halo_ = arena_.take<int>(nhalo_all);
+    edges = arena_.take<EdgeRec<R>>(nedge_all);
+    d_edges_ = edges;
+    d_tiles_ = arena_.take<TileDesc>(tiles.size());
+    w0 = arena_.take<SVec4<R>>(ncell_state);
+    d_w0_ = w0;
+    d_wc_ = arena_.take<SVec4<R>>(ncell_state);
+    d_res_ = arena_.take<SVec4<R>>(ncell_state);
+    d_dt_ = arena_.take<R>(ncell_state);
+    d_partial_ = arena_.take<double>(tiles.size());
+    d_tile_min_ = arena_.take<R>(tiles.size());
+    d_dtg_ = arena_.take<R>(ncase);
+    wcell = arena_.take<int>(nwall_all);
+    wvx = arena_.take<double>(nwall_all);
+    wvy = arena_.take<double>(nwall_all);
+    wvn = arena_.take<double>(nwall_all);
+    d_case_st0_ = arena_.take<int>(ncase);
+    d_case_w0_ = arena_.take<int>(ncase);
+    d_case_nw_ = arena_.take<int>(ncase);
+    d_cl_ = arena_.take<double>(ncase);
+    d_cd_ = arena_.take<double>(ncase);
+    walls_ = WallTable{wcell, wvx, wvy, wvn};
+  };
+  carve();
+  arena_.allocate();
+  carve();
+
+  // host images
+  std::vector<FlowT<R>> hflow(ncase);
+  std::vector<FlowT<double>> hflow64(ncase);
+  for (int c = 0; c < ncase; c++) {
+    hflow[c] = cast_flow<R>(cases_[c].f64);
+    hflow64[c] = cases_[c].f64;
+  }
+  std::vector<CellGeom<R>> hcg(ncell_all);
+  std::vector<int4> hnbr(ncell_all), hslot(ncell_all), hnloc(ncell_all);
+  std::vector<int> hhalo(nhalo_all);
+  std::vector<EdgeRec<R>> hedge(nedge_all);
+  std::vector<int> hwcell(nwall_all);
+  std::vector<double> hwvx(nwall_all), hwvy(nwall_all), hwvn(nwall_all);
+  for (int m = 0; m < nmesh; m++) {
+    const TiledMesh& T = tiled_[m];
+    for (int i = 0; i < T.ncell; i++) {
+      CellGeom<R>& g = hcg[geo0[m] + i];
+      for (int k = 0; k < 3; k++) {
+        g.vx[k] = (R)T.gvx[3 * i + k];
+        g.vy[k] = (R)T.gvy[3 * i + k];
+      }
+      g.inv_area = (R)T.inv_area[i];
+      g.area = (R)T.area[i];
+      hnbr[geo0[m] + i] = make_int4(T.nbr[3 * i], T.nbr[3 * i + 1], T.nbr[3 * i + 2], T.clog[i]);
+      hslot[geo0[m] + i] = make_int4(T.slot[3 * i], T.slot[3 * i + 1], T.slot[3 * i + 2], 0);
+      hnloc[geo0[m] + i] = make_int4(T.nloc[3 * i], T.nloc[3 * i + 1], T.nloc[3 * i + 2], 0);
+    }
+    std::copy(T.halo.begin(), T.halo.end(), hhalo.begin() + halo0[m]);
+    for (size_t s = 0; s < T.e_l.size(); s++) {
+      EdgeRec<R>& e = hedge[edge0[m] + s];
+      e.nx = (R)T.e_nx[s];
+      e.ny = (R)T.e_ny[s];
+      e.vn = (R)T.e_vn[s];
+      e.vx = (R)T.e_vx[s];
+      e.vy = (R)T.e_vy[s];
+      e.rlx = (R)T.e_rlx[s];
+      e.rly = (R)T.e_rly[s];
+      e.rrx = (R)T.e_rrx[s];
+      e.rry = (R)T.e_rry[s];
+      e.l = T.e_l[s];
+      e.r = T.e_r[s];
+    }
+    for (size_t w = 0; w < T.w_cell.size(); w++) {
+      hwcell[wall0[m] + w] = T.w_cell[w];
+      hwvx[wall0[m] + w] = T.w_vx[w];
+      hwvy[wall0[m] + w] = T.w_vy[w];
+      hwvn[wall0[m] + w] = T.w_vn[w];
+    }
+  }
+  std::vector<SVec4<R>> hw0(ncell_state);
+  std::vector<int> hst0(ncase), hw0off(ncase), hnw(ncase);
+  for (int c = 0; c < ncase; c++) {
+    const int m = cases_[c].mesh;
+    const double al = cases_[c].flow.alpha_deg * kPI / 180.0;
+    const SVec4<R> fs{(R)1.0, (R)std::cos(al), (R)std::sin(al), (R)cases_[c].rhoE_inf};
+    for (int i = 0; i < tiled_[m].ncell; i++) hw0[state_off_[c] + i] = fs;
+    hst0[c] = state_off_[c];
+    hw0off[c] = wall0[m];
+    hnw[c] = (int)tiled_[m].w_cell.size();
+  }
+  h2d(flow, hflow, stream_);
+  h2d(d_flow64_, hflow64, stream_);
+  h2d(cg, hcg, stream_);
+  h2d(d_nbr_, hnbr, stream_);
+  h2d(d_slot_, hslot, stream_);
+  h2d(d_nloc_, hnloc, stream_);
+  h2d(d_halo_, hhalo, stream_);
+  h2d(edges, hedge, stream_);
+  h2d(d_tiles_, tiles, stream_);
+  h2d(w0, hw0, stream_);
+  h2d(wcell, hwcell, stream_);
+  h2d(wvx, hwvx, stream_);
+  h2d(wvy, hwvy, stream_);
+  h2d(wvn, hwvn, stream_);
+  h2d(d_case_st0_, hst0, stream_);
+  h2d(d_case_w0_, hw0off, stream_);
+  h2d(d_case_nw_, hnw, stream_);
+  h2d(d_ctl_, h_ctl_, stream_);
+  HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void Batch::setup_plain() {
+  const int ncase = (int)cases_.size();
+  h_ctl_.assign(ncase, CaseCtl{});
+  plain_.resize(ncase);
+  plain_walls_.resize(ncase);
+  int nblk_all = 0;
+  for (int c = 0; c < ncase; c++) {
+    const Topology& t = topo_[cases_[c].mesh];
+    const size_t C = t.ncell, E = t.nedge, nw = std::max(t.nwall, 1), nf = std::max(t.nfar, 1);
+    const int nblk = t.ncell / 64 + 1;  // gpu_solver.cu:2217 block count at 64 threads
+    h_ctl_[c].tile0 = nblk_all;
+    h_ctl_[c].ntile = nblk;
+    h_ctl_[c].resd = 100.0;
+    h_ctl_[c].status = FVM_OK;
+    nblk_all += nblk;
+  }
+  auto carve = [&]() {
+    for (int c = 0; c < ncase; c++) {
+      const Topology& t = topo_[cases_[c].mesh];
+      PlainDev& p = plain_[c];
+      const size_t C = t.ncell, E = t.nedge, nw = std::max(t.nwall, 1), nf = std::max(t.nfar, 1);
+      p.C = t.ncell;
+      p.E = t.nedge;
+      p.nfar = t.nfar;
+      p.nwall = t.nwall;
+      p.nblk64 = t.ncell / 64 + 1;
+      for (int q = 0; q < 4; q++) p.w[q] = arena_.take<double>(C);
+      p.P = arena_.take<double>(C);
+      p.A = arena_.take<double>(C);
+      for (int q = 0; q < 4; q++) p.w0[q] = arena_.take<double>(C);
+      for (int q = 0; q < 4; q++) p.farw[q] = arena_.take<double>(nf);
+      for (int q = 0; q < 4; q++) p.wallw[q] = arena_.take<double>(nw);
+      for (int k = 0; k < 3; k++) p.gvx[k] = arena_.take<double>(C);
+      for (int k = 0; k < 3; k++) p.gvy[k] = arena_.take<double>(C);
+      for (int k = 0; k < 3; k++) p.celledge[k] = arena_.take<int>(C);
+      for (int k = 0; k < 3; k++) p.neicell[k] = arena_.take<int>(C);
+      p.clog = arena_.take<int>(C);
+      p.area = arena_.take<double>(C);
+      p.cx = arena_.take<double>(C);
+      p.cy = arena_.take<double>(C);
+      for (int q = 0; q < 8; q++) p.grad[q] = arena_.take<double>(C);
+      for (int q = 0; q < 4; q++) p.umax[q] = arena_.take<double>(C);
+      for (int q = 0; q < 4; q++) p.umin[q] = arena_.take<double>(C);
+      p.elog = arena_.take<int>(E);
+      p.left = arena_.take<int>(E);
+      p.right = arena_.take<int>(E);
+      p.farid = arena_.take<int>(E);
+      p.wallid = arena_.take<int>(E);
+      p.vx = arena_.take<double>(E);
+      p.vy = arena_.take<double>(E);
+      p.vn = arena_.take<double>(E);
+      p.midx = arena_.take<double>(E);
+      p.midy = arena_.take<double>(E);
+      for (int q = 0; q < 4; q++) p.rel[q] = arena_.take<double>(E);
+      for (int q = 0; q < 4; q++) p.rer[q] = arena_.take<double>(E);
+      p.dtl = arena_.take<double>(E);
+      p.dtr = arena_.take<double>(E);
+      for (int q = 0; q < 4; q++) p.res[q] = arena_.take<double>(C);
+      for (int q = 0; q < 4; q++) p.r1[q] = arena_.take<double>(C);
+      p.dt = arena_.take<double>(C);
+      p.tsum = arena_.take<double>(C);
+      p.blockmin = arena_.take<double>((C + 255) / 256 + 1);
+      p.dtg = arena_.take<double>(1);
+      int* wc = arena_.take<int>(t.nwall);
+      double* wx = arena_.take<double>(t.nwall);
+      double* wy = arena_.take<double>(t.nwall);
+      double* wn = arena_.take<double>(t.nwall);
+      plain_walls_[c] = WallTable{wc, wx, wy, wn};
+    }
+    d_ctl_ = arena_.take<CaseCtl>(ncase);
+    d_flow64_ = arena_.take<FlowT<double>>(ncase);
+    d_partial_ = arena_.take<double>(nblk_all);
+    d_cl_ = arena_.take<double>(ncase);
+    d_cd_ = arena_.take<double>(ncase);
+  };
+  carve();
+  arena_.allocate();
+  carve();
+  for (int c = 0; c < ncase; c++) {
+    const Topology& t = topo_[cases_[c].mesh];
+    PlainDev& p = plain_[c];
+    const size_t C = t.ncell;
+    const WallTable& wt = plain_walls_[c];
+    int* wc = const_cast<int*>(wt.cell);
+    double* wx = const_cast<double*>(wt.vx);
+    double* wy = const_cast<double*>(wt.vy);
+    double* wn = const_cast<double*>(wt.vn);
+    // host images in reference numbering
+    std::vector<double> gx[3], gy[3];
+    std::vector<int> ce[3], nc[3];
+    for (int k = 0; k < 3; k++) {
+      gx[k].resize(C);
+      gy[k].resize(C);
+      ce[k].resize(C);
+      nc[k].resize(C);
+    }
+    for (size_t i = 0; i < C; i++)
+      for (int k = 0; k < 3; k++) {
+        const int N1 = t.pt[3 * i + (k + 1) % 3], N2 = t.pt[3 * i + (k + 2) % 3];
+        gx[k][i] = t.x[N1] - t.x[N2];
+        gy[k][i] = t.y[N2] - t.y[N1];
+        ce[k][i] = t.celledge[3 * i + k];
+        nc[k][i] = t.neicell[3 * i + k];
+      }
+    for (int k = 0; k < 3; k++) {
+      h2d(p.gvx[k], gx[k], stream_);
+      h2d(p.gvy[k], gy[k], stream_);
+      h2d(p.celledge[k], ce[k], stream_);
+      h2d(p.neicell[k], nc[k], stream_);
+    }
+    h2d(p.clog, t.clog, stream_);
+    h2d(p.area, t.area, stream_);
+    h2d(p.cx, t.cx, stream_);
+    h2d(p.cy, t.cy, stream_);
+    h2d(p.elog, t.elog, stream_);
+    h2d(p.left, t.left, stream_);
+    h2d(p.right, t.right, stream_);
+    h2d(p.farid, t.farid, stream_);
+    h2d(p.wallid, t.wallid, stream_);
+    h2d(p.vx, t.vx, stream_);
+    h2d(p.vy, t.vy, stream_);
+    h2d(p.vn, t.vn, stream_);
+    h2d(p.midx, t.midx, stream_);
+    h2d(p.midy, t.midy, stream_);
+    std::vector<int> hwc(t.nwall);
+    std::vector<double> hwx(t.nwall), hwy(t.nwall), hwn(t.nwall);
+    for (int w = 0; w < t.nwall; w++) {
+      const int e = t.wall_edge[w];
+      hwc[w] = t.left[e];
+      hwx[w] = t.vx[e];
+      hwy[w] = t.vy[e];
+      hwn[w] = t.vn[e];
+    }
+    h2d(wc, hwc, stream_);
+    h2d(wx, hwx, stream_);
+    h2d(wy, hwy, stream_);
+    h2d(wn, hwn, stream_);
+    // freestream state
+    const double al = cases_[c].flow.alpha_deg * kPI / 180.0;
+    std::vector<double> s0(C, 1.0), s1(C, std::cos(al)), s2(C, std::sin(al)), s3(C, cases_[c].rhoE_inf);
+    h2d(p.w[0], s0, stream_);
+    h2d(p.w[1], s1, stream_);
+    h2d(p.w[2], s2, stream_);
+    h2d(p.w[3], s3, stream_);
+  }
+  for (int c = 0; c < ncase; c++) plain_[c].partial = d_partial_ + h_ctl_[c].tile0;
+  std::vector<FlowT<double>> hflow(ncase);
+  for (int c = 0; c < ncase; c++) hflow[c] = cases_[c].f64;
+  h2d(d_flow64_, hflow, stream_);
+  h2d(d_ctl_, h_ctl_, stream_);
+  tile_cells_ = 0;
+  threads_ = 256;
+  total_tiles_ = 0;
+  HIP_OK(hipStreamSynchronize(stream_));
+}
+
+template <typename R>
+TiledArgs<R> Batch::tiled_args() {
+  TiledArgs<R> a{};
+  a.tiles = d_tiles_;
+  a.ntiles = total_tiles_;
+  a.ctl = d_ctl_;
+  a.flow = static_cast<const FlowT<R>*>(d_flow_);
+  a.cg = static_cast<const CellGeom<R>*>(d_cg_);
+  a.nbr = d_nbr_;
+  a.slot = d_slot_;
+  a.nloc = d_nloc_;
+  a.halo = d_halo_;
+  a.edges = static_cast<const EdgeRec<R>*>(d_edges_);
+  a.res = static_cast<SVec4<R>*>(d_res_);
+  a.dt = static_cast<R*>(d_dt_);
+  a.tile_min = static_cast<R*>(d_tile_min_);
+  a.dt_global = static_cast<const R*>(d_dtg_);
+  a.w0 = static_cast<const SVec4<R>*>(d_w0_);
+  a.partial = d_partial_;
+  a.order = o_.order;
+  a.smoothing = o_.smoothing;
+  a.global_dt = o_.global_dt;
+  int ML = 0, ME = 0;
+  for (auto& t : tiled_) {
+    ML = std::max(ML, t.max_local);
+    ME = std::max(ME, t.max_edges);
+  }
+  a.max_local = ML;
+  a.max_edges = ME;
+  return a;
+}
+
+template <typename R>
+void Batch::enqueue_tiled_steps(int nsteps) {
+  TiledArgs<R> a = tiled_args<R>();
+  SVec4<R>* W0 = static_cast<SVec4<R>*>(d_w0_);
+  SVec4<R>* Wc = static_cast<SVec4<R>*>(d_wc_);
+  for (int s = 0; s < nsteps; s++) {
+    for (int k = 0; k < 4; k++) {
+      a.w_in = k == 0 ? W0 : Wc;
+      a.w_out = k == 3 ? W0 : Wc;
+      a.rk = (R)kRK[k];
+      a.last_stage = k == 3;
+      HIP_OK(launch_flux_tile<R>(a, threads_, lds_a_, stream_));
+      if (o_.global_dt)
+        HIP_OK(launch_global_dt_min<R>(d_ctl_, static_cast<R*>(d_tile_min_),
+                                       static_cast<R*>(d_dtg_), (int)cases_.size(), stream_));
+      HIP_OK(launch_update_tile<R>(a, threads_, lds_b_, stream_));
+    }
+    hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(64), 0, stream_, d_ctl_,
+                       d_partial_, d_hist_, hist_stride_, 0);
+    HIP_OK(hipGetLastError());
+  }
+}
+
+template <typename R>
+void Batch::time_tiled(int reps, double ms[2]) {
+  // Second-stage arguments: reads Wc and W0, writes R, dt and Wc only; the
+  // update is W0 - a dt R / A of unchanged inputs, so repeating it is
+  // idempotent and the next stage overwrites Wc anyway.
+  TiledArgs<R> a = tiled_args<R>();
+  a.w_in = static_cast<SVec4<R>*>(d_wc_);
+  a.w_out = static_cast<SVec4<R>*>(d_wc_);
+  a.rk = (R)kRK[1];
+  a.last_stage = 0;
+  // every case active for the measurement, then restore the control block
+  std::vector<CaseCtl> on = h_ctl_;
+  for (auto& c : on) c.active = 1;
+  HIP_OK(hipMemcpyAsync(d_ctl_, on.data(), on.size() * sizeof(CaseCtl), hipMemcpyHostToDevice, stream_));
+  HIP_OK(launch_flux_tile<R>(a, threads_, lds_a_, stream_));
+  HIP_OK(launch_update_tile<R>(a, threads_, lds_b_, stream_));
+  for (int k = 0; k < 2; k++) {
+    HIP_OK(hipEventRecord(ev0_, stream_));
+    for (int r = 0; r < reps; r++) {
+      if (k == 0)
+        HIP_OK(launch_flux_tile<R>(a, threads_, lds_a_, stream_));
+      else
+        HIP_OK(launch_update_tile<R>(a, threads_, lds_b_, stream_));
+    }
+    HIP_OK(hipEventRecord(ev1_, stream_));
+    HIP_OK(hipEventSynchronize(ev1_));
+    float t = 0.f;
+    HIP_OK(hipEventElapsedTime(&t, ev0_, ev1_));
+    ms[k] = (double)t / reps;
+  }
+  HIP_OK(hipMemcpyAsync(d_ctl_, h_ctl_.data(), h_ctl_.size() * sizeof(CaseCtl), hipMemcpyHostToDevice, stream_));
+  HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void Batch::kernel_times(int reps, double ms[2]) {
+  HIP_OK(hipSetDevice(device_));
+  if (o_.kernels != FVM_KERNELS_TILED) throw Error{FVM_ERR_ARGUMENT, "kernel timing covers the tiled kernels only"};
+  if (reps < 1) throw Error{FVM_ERR_ARGUMENT, "reps must be >= 1"};
+  if (o_.precision == FVM_FP64)
+    time_tiled<double>(reps, ms);
+  else
+    time_tiled<float>(reps, ms);
+}
+
+void Batch::enqueue_plain_steps(int nsteps) {
+  for (int s = 0; s < nsteps; s++) {
+    for (int c = 0; c < (int)cases_.size(); c++)
+      plain_launch_step(plain_[c], d_ctl_, c, d_flow64_, o_.order, o_.smoothing, o_.global_dt,
+                        stream_);
+    HIP_OK(hipGetLastError());
+    hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(64), 0, stream_, d_ctl_,
+                       d_partial_, d_hist_, hist_stride_, 1);
+    HIP_OK(hipGetLastError());
+  }
+}
+
+void Batch::enqueue_steps(int nsteps) {
+  if (o_.kernels == FVM_KERNELS_PLAIN)
+    enqueue_plain_steps(nsteps);
+  else if (o_.precision == FVM_FP64)
+    enqueue_tiled_steps<double>(nsteps);
+  else
+    enqueue_tiled_steps<float>(nsteps);
+}
+
+void Batch::build_graph() {
+  if (graph_) {
+    HIP_OK(hipGraphExecDestroy(graph_));
+    graph_ = nullptr;
+  }
+  hipGraph_t g;
+  HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  try {
+    enqueue_steps(steps_per_graph_);
+  } catch (...) {
+    (void)hipStreamEndCapture(stream_, &g);
+    throw;
+  }
+  HIP_OK(hipStreamEndCapture(stream_, &g));
+  HIP_OK(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+  HIP_OK(hipGraphDestroy(g));
+}
+
+void Batch::run(int max_steps, double epsl, double* hist, int* steps_done, int* case_status) {
+  HIP_OK(hipSetDevice(device_));
+  if (max_steps < 0) throw Error{FVM_ERR_ARGUMENT, "max_steps must be >= 0"};
+  const int ncase = (int)cases_.size();
+  // history buffer: one row of max_steps+1 per case (the graph captures its address)
+  if (max_steps + 1 > hist_stride_ || !d_hist_) {
+    if (d_hist_) HIP_OK(hipFree(d_hist_));
+    d_hist_ = nullptr;
+    hist_stride_ = std::max(max_steps + 1, 2);
+    HIP_OK(hipMalloc(&d_hist_, (size_t)ncase * hist_stride_ * sizeof(double)));
+    if (graph_) {
+      HIP_OK(hipGraphExecDestroy(graph_));
+      graph_ = nullptr;
+    }
+  }
+  std::vector<int> before(ncase);
+  for (int c = 0; c < ncase; c++) {
+    CaseCtl& k = h_ctl_[c];
+    before[c] = k.step;
+    k.max_steps = max_steps;
+    k.epsl = epsl;
+    k.active = (k.step < max_steps && k.resd > epsl && k.status == FVM_OK) ? 1 : 0;
+  }
+  if (!graph_) build_graph();
+  HIP_OK(hipMemcpyAsync(d_ctl_, h_ctl_.data(), ncase * sizeof(CaseCtl), hipMemcpyHostToDevice, stream_));
+  HIP_OK(hipEventRecord(ev0_, stream_));
+  for (;;) {
+    int min_step = max_steps, any = 0;
+    for (int c = 0; c < ncase; c++)
+      if (h_ctl_[c].active) {
+        any = 1;
+        min_step = std::min(min_step, h_ctl_[c].step);
+      }
+    if (!any) break;
+    const int remaining = max_steps - min_step;
+    if (remaining >= steps_per_graph_)
+      HIP_OK(hipGraphLaunch(graph_, stream_));
+    else
+      enqueue_steps(remaining);
+    HIP_OK(hipMemcpyAsync(h_ctl_.data(), d_ctl_, ncase * sizeof(CaseCtl), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+  }
+  HIP_OK(hipEventRecord(ev1_, stream_));
+  HIP_OK(hipEventSynchronize(ev1_));
+  float ms = 0.f;
+  HIP_OK(hipEventElapsedTime(&ms, ev0_, ev1_));
+  last_ms = ms;
+  last_csu = 0;
+  for (int c = 0; c < ncase; c++)
+    last_csu += 4LL * ncell(c) * (int64_t)(h_ctl_[c].step - before[c]);
+  if (hist) {
+    std::vector<double> h((size_t)ncase * hist_stride_, 0.0);
+    HIP_OK(hipMemcpy(h.data(), d_hist_, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int c = 0; c < ncase; c++) {
+      double* row = hist + (size_t)c * (max_steps + 1);
+      row[0] = 100.0;
+      for (int s = before[c] + 1; s <= h_ctl_[c].step && s <= max_steps; s++)
+        row[s] = h[(size_t)c * hist_stride_ + s];
+    }
+  }
+  for (int c = 0; c < ncase; c++) {
+    if (steps_done) steps_done[c] = h_ctl_[c].step;
+    if (case_status) case_status[c] = h_ctl_[c].status;
+  }
+}
+
+void Batch::get_state(int icase, double* rho, double* rhou, double* rhov, double* rhoe) {
+  HIP_OK(hipSetDevice(device_));
+  if (icase < 0 || icase >= ncase()) throw Error{FVM_ERR_ARGUMENT, "case index out of range"};
+  const int C = ncell(icase);
+  double* out[4] = {rho, rhou, rhov, rhoe};
+  if (o_.kernels == FVM_KERNELS_PLAIN) {
+    for (int q = 0; q < 4; q++)
+      if (out[q])
+        HIP_OK(hipMemcpy(out[q], plain_[icase].w[q], C * sizeof(double), hipMemcpyDeviceToHost));
+    return;
+  }
+  const TiledMesh& T = tiled_[cases_[icase].mesh];
+  auto fetch = [&](auto tag) {
+    using R = decltype(tag);
+    std::vector<SVec4<R>> h(C);
+    HIP_OK(hipMemcpy(h.data(), static_cast<SVec4<R>*>(d_w0_) + state_off_[icase],
+                     C * sizeof(SVec4<R>), hipMemcpyDeviceToHost));
+    for (int n = 0; n < C; n++) {
+      const int o = T.new_to_old[n];
+      if (rho) rho[o] = (double)h[n].d;
+      if (rhou) rhou[o] = (double)h[n].du;
+      if (rhov) rhov[o] = (double)h[n].dv;
+      if (rhoe) rhoe[o] = (double)h[n].de;
+    }
+  };
+  if (o_.precision == FVM_FP64)
+    fetch(double{});
+  else
+    fetch(float{});
+}
+
+void Batch::set_state(int icase, const double* rho, const double* rhou, const double* rhov,
+                      const double* rhoe) {
+  HIP_OK(hipSetDevice(device_));
+  if (icase < 0 || icase >= ncase()) throw Error{FVM_ERR_ARGUMENT, "case index out of range"};
+  if (!rho || !rhou || !rhov || !rhoe) throw Error{FVM_ERR_ARGUMENT, "set_state: NULL array"};
+  const int C = ncell(icase);
+  if (o_.kernels == FVM_KERNELS_PLAIN) {
+    const double* in[4] = {rho, rhou, rhov, rhoe};
+    for (int q = 0; q < 4; q++)
+      HIP_OK(hipMemcpy(plain_[icase].w[q], in[q], C * sizeof(double), hipMemcpyHostToDevice));
+    return;
+  }
+  const TiledMesh& T = tiled_[cases_[icase].mesh];
+  auto put = [&](auto tag) {
+    using R = decltype(tag);
+    std::vector<SVec4<R>> h(C);
+    for (int n = 0; n < C; n++) {
+      const int o = T.new_to_old[n];
+      h[n] = SVec4<R>{(R)rho[o], (R)rhou[o], (R)rhov[o], (R)rhoe[o]};
+    }
+    HIP_OK(hipMemcpy(static_cast<SVec4<R>*>(d_w0_) + state_off_[icase], h.data(),
+                     C * sizeof(SVec4<R>), hipMemcpyHostToDevice));
+  };
+  if (o_.precision == FVM_FP64)
+    put(double{});
+  else
+    put(float{});
+}
+
+void Batch::forces(double* cl, double* cd) {
+  HIP_OK(hipSetDevice(device_));
+  const int ncase = (int)cases_.size();
+  if (o_.kernels == FVM_KERNELS_PLAIN) {
+    for (int c = 0; c < ncase; c++)
+      hipLaunchKernelGGL(forces_plain, dim3(1), dim3(64), 0, stream_, plain_[c], plain_walls_[c],
+                         topo_[cases_[c].mesh].nwall, d_flow64_ + c, d_cl_ + c, d_cd_ + c);
+  } else if (o_.precision == FVM_FP64) {
+    hipLaunchKernelGGL(forces_tiled<double>, dim3(ncase), dim3(64), 0, stream_,
+                       static_cast<const SVec4<double>*>(d_w0_), d_case_st0_, d_case_w0_, d_case_nw_,
+                       walls_, d_flow64_, d_cl_, d_cd_);
+  } else {
+    hipLaunchKernelGGL(forces_tiled<float>, dim3(ncase), dim3(64), 0, stream_,
+                       static_cast<const SVec4<float>*>(d_w0_), d_case_st0_, d_case_w0_, d_case_nw_,
+                       walls_, d_flow64_, d_cl_, d_cd_);
+  }
+  HIP_OK(hipGetLastError());
+  if (cl) HIP_OK(hipMemcpyAsync(cl, d_cl_, ncase * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  if (cd) HIP_OK(hipMemcpyAsync(cd, d_cd_, ncase * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIP_OK(hipStreamSynchronize(stream_));
+}
+
+}  // namespace fvm
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using fvm::Batch;
+using fvm::Error;
+
+struct fvm_batch {
+  std::unique_ptr<Batch> b;
+};
+struct fvm_solver {
+  std::unique_ptr<Batch> b;
+};
+
+template <typename F>
+static fvm_status guard(F&& f) {
+  try {
+    fvm::set_last_error("");
+    f();
+    return FVM_OK;
+  } catch (const Error& e) {
+    fvm::set_last_error(e.msg);
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    fvm::set_last_error("host allocation failed");
+    return FVM_ERR_ALLOC;
+  } catch (const std::exception& e) {
+    fvm::set_last_error(e.what());
+    return FVM_ERR_ARGUMENT;
+  }
+}
+
+extern "C" {
+
+void fvm_default_opts(fvm_scheme_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->order = 2;
+  o->smoothing = 1;
+  o->eps_smooth = 0.8;  // gpu_solver.cu:1633
+  o->global_dt = 0;
+  o->precision = FVM_FP64;
+  o->kernels = FVM_KERNELS_TILED;
+  o->device = 0;
+}
+
+const char* fvm_status_string(fvm_status s) {
+  switch (s) {
+    case FVM_OK: return "ok";
+    case FVM_ERR_ARGUMENT: return "invalid argument";
+    case FVM_ERR_MESH: return "invalid mesh";
+    case FVM_ERR_AREA: return "non-positive cell area";
+    case FVM_ERR_HIP: return "HIP runtime error";
+    case FVM_ERR_NAN: return "non-finite residual";
+    case FVM_ERR_NOT_CONVERGED: return "not converged";
+    case FVM_ERR_NO_DEVICE: return "no HIP device";
+    case FVM_ERR_ALLOC: return "allocation failed";
+  }
+  return "unknown status";
+}
+
+const char* fvm_last_error(void) { return fvm::g_last_error.c_str(); }
+
+fvm_status fvm_mesh_topology(const fvm_mesh_desc* mesh, int32_t counts[5], int32_t* celledge,
+                             int32_t* neicell, int32_t* clog, int32_t* elog, int32_t* left,
+                             int32_t* right, int32_t* farid, int32_t* wallid, int32_t* node1,
+                             int32_t* node2) {
+  return guard([&] {
+    if (!mesh || !counts) throw Error{FVM_ERR_ARGUMENT, "mesh and counts must not be NULL"};
+    const fvm::Topology t = fvm::build_topology(*mesh);
+    counts[0] = t.nnode;
+    counts[1] = t.ncell;
+    counts[2] = t.nedge;
+    counts[3] = t.nwall;
+    counts[4] = t.nfar;
+    for (int i = 0; i < 3 * t.ncell; i++) {
+      if (celledge) celledge[i] = t.celledge[i] + 1;
+      if (neicell) neicell[i] = t.neicell[i] >= 0 ? t.neicell[i] + 1 : t.neicell[i];
+    }
+    for (int c = 0; c < t.ncell; c++)
+      if (clog) clog[c] = t.clog[c];
+    for (int e = 0; e < t.nedge; e++) {
+      if (elog) elog[e] = t.elog[e];
+      if (left) left[e] = t.left[e] + 1;
+      if (right) right[e] = t.right[e] >= 0 ? t.right[e] + 1 : -1;
+      if (farid) farid[e] = t.farid[e];
+      if (wallid) wallid[e] = t.wallid[e];
+      if (node1) node1[e] = t.node1[e] + 1;
+      if (node2) node2[e] = t.node2[e] + 1;
+    }
+  });
+}
+
+fvm_status fvm_batch_create(int32_t nmesh, const fvm_mesh_desc* meshes, int32_t ncase,
+                            const int32_t* case_mesh, const fvm_flow_params* flows,
+                            const fvm_scheme_opts* opts, fvm_batch** out) {
+  return guard([&] {
+    if (!out) throw Error{FVM_ERR_ARGUMENT, "out must not be NULL"};
+    *out = nullptr;
+    fvm_scheme_opts o;
+    if (opts)
+      o = *opts;
+    else
+      fvm_default_opts(&o);
+    auto h = std::make_unique<fvm_batch>();
+    h->b = std::make_unique<Batch>(nmesh, meshes, ncase, case_mesh, flows, o);
+    *out = h.release();
+  });
+}
+
+fvm_status fvm_batch_run(fvm_batch* b, int32_t max_steps, double epsl, double* resid_hist,
+                         int32_t* steps_done, int32_t* case_status) {
+  std::vector<int32_t> cs;
+  fvm_status st = guard([&] {
+    if (!b) throw Error{FVM_ERR_ARGUMENT, "NULL batch"};
+    cs.assign(b->b->ncase(), FVM_OK);
+    b->b->run(max_steps, epsl, resid_hist, steps_done, cs.data());
+  });
+  if (st != FVM_OK) return st;
+  if (case_status) std::copy(cs.begin(), cs.end(), case_status);
+  for (int32_t s : cs)
+    if (s != FVM_OK) {
+      fvm::set_last_error("at least one case's residual became non-finite");
+      return (fvm_status)s;
+    }
+  return FVM_OK;
+}
+
+fvm_status fvm_batch_get_state(fvm_batch* b, int32_t icase, double* rho, double* rhou, double* rhov,
+                               double* rhoE) {
+  return guard([&] {
+    if (!b) throw Error{FVM_ERR_ARGUMENT, "NULL batch"};
+    b->b->get_state(icase, rho, rhou, rhov, rhoE);
+  });
+}
+
+fvm_status fvm_batch_get_forces(fvm_batch* b, double* cl, double* cd) {
+  return guard([&] {
+    if (!b) throw Error{FVM_ERR_ARGUMENT, "NULL batch"};
+    b->b->forces(cl, cd);
+  });
+}
+
+fvm_status fvm_batch_get_timing(fvm_batch* b, double* ms, int64_t* csu) {
+  return guard([&] {
+    if (!b) throw Error{FVM_ERR_ARGUMENT, "NULL batch"};
+    if (ms) *ms = b->b->last_ms;
+    if (csu) *csu = b->b->last_csu;
+  });
+}
+
+fvm_status fvm_batch_get_config(fvm_batch* b, int32_t config[4]) {
+  return guard([&] {
+    if (!b || !config) throw Error{FVM_ERR_ARGUMENT, "NULL argument"};
+    b->b->config(config);
+  });
+}
+
+fvm_status fvm_batch_kernel_times(fvm_batch* b, int32_t reps, double ms[2]) {
+  return guard([&] {
+    if (!b || !ms) throw Error{FVM_ERR_ARGUMENT, "NULL argument"};
+    b->b->kernel_times(reps, ms);
+  });
+}
+
+fvm_status fvm_kernel_times(fvm_solver* s, int32_t reps, double ms[2]) {
+  return guard([&] {
+    if (!s || !ms) throw Error{FVM_ERR_ARGUMENT, "NULL argument"};
+    s->b->kernel_times(reps, ms);
+  });
+}
+
+void fvm_batch_destroy(fvm_batch* b) { delete b; }
+
+fvm_status fvm_create(const fvm_mesh_desc* mesh, const fvm_flow_params* flow,
+                      const fvm_scheme_opts* opts, fvm_solver** out) {
+  return guard([&] {
+    if (!out || !mesh || !flow) throw Error{FVM_ERR_ARGUMENT, "NULL argument"};
+    *out = nullptr;
+    fvm_scheme_opts o;
+    if (opts)
+      o = *opts;
+    else
+      fvm_default_opts(&o);
+    const int32_t zero = 0;
+    auto h = std::make_unique<fvm_solver>();
+    h->b = std::make_unique<Batch>(1, mesh, 1, &zero, flow, o);
+    *out = h.release();
+  });
+}
+
+fvm_status fvm_run(fvm_solver* s, int32_t max_steps, double epsl, double* resid_hist,
+                   int32_t* steps_done) {
+  int32_t status = FVM_OK;
+  fvm_status st = guard([&] {
+    if (!s) throw Error{FVM_ERR_ARGUMENT, "NULL solver"};
+    s->b->run(max_steps, epsl, resid_hist, steps_done, &status);
+  });
+  if (st != FVM_OK) return st;
+  if (status != FVM_OK) fvm::set_last_error("the residual became non-finite");
+  return (fvm_status)status;
+}
+
+fvm_status fvm_get_state(fvm_solver* s, double* rho, double* rhou, double* rhov, double* rhoE) {
+  return guard([&] {
+    if (!s) throw Error{FVM_ERR_ARGUMENT, "NULL solver"};
+    s->b->get_state(0, rho, rhou, rhov, rhoE);
+  });
+}
+
+fvm_status fvm_set_state(fvm_solver* s, const double* rho, const double* rhou, const double* rhov,
+                         const double* rhoE) {
+  return guard([&] {
+    if (!s) throw Error{FVM_ERR_ARGUMENT, "NULL solver"};
+    s->b->set_state(0, rho, rhou, rhov, rhoE);
+  });
+}
+
+fvm_status fvm_get_forces(fvm_solver* s, double* cl, double* cd) {
+  return guard([&] {
+    if (!s) throw Error{FVM_ERR_ARGUMENT, "NULL solver"};
+    s->b->forces(cl, cd);
+  });
+}
+
+fvm_status fvm_get_timing(fvm_solver* s, double* ms, int64_t* csu) {
+  return guard([&] {
+    if (!s) throw Error{FVM_ERR_ARGUMENT, "NULL solver"};
+    if (ms) *ms = s->b->last_ms;
+    if (csu) *csu = s->b->last_csu;
+  });
+}
+
+void fvm_destroy(fvm_solver* s) { delete s; }
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+// Internal data structures of libfvm_hip.so: host topology, tiled device
+// layout, and the kernel argument blocks shared by the launch code and the
+// kernels.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fvm_abi.h"
+
+namespace fvm {
+
+// Error carrier used inside the library; the C-ABI turns it into a status.
+struct Error {
+  fvm_status status;
+  std::string msg;
+};
+void set_last_error(const std::string& msg);
+
+// ---------------------------------------------------------------------------
+// Topology and FP64 geometry in the reference's conventions (0-based storage;
+// edge order after the interior / far / wall partition).
+// ---------------------------------------------------------------------------
+struct Topology {
+  int nnode = 0, ncell = 0, nedge = 0, nwall = 0, nfar = 0;
+  std::vector<double> x, y;          // [nnode]
+  std::vector<int> pt;               // [3*ncell] 0-based node ids
+  std::vector<double> area, cx, cy;  // [ncell]
+  std::vector<int> celledge;         // [3*ncell] 0-based edge id
+  std::vector<int> neicell;          // [3*ncell] 0-based cell, or -1 far / -2 wall
+  std::vector<int> clog;             // [ncell]
+  std::vector<int> elog, left, right, farid, wallid, node1, node2;  // [nedge]; right -1 on boundary,
+                                                                     // farid/wallid 1-based, nodes 0-based
+  std::vector<double> vx, vy, vn, midx, midy;                        // [nedge]
+  std::vector<int> wall_edge;        // [nwall] edge id in wall-id order
+};
+
+// Throws Error.  Rejects out-of-range node ids, non-manifold edges and
+// non-positive cell areas.
+Topology build_topology(const fvm_mesh_desc& mesh);
+
+// ---------------------------------------------------------------------------
+// Tiled layout used by the tiled kernels.  Cells are renumbered by recursive
+// coordinate bisection into tiles of at most tile_cells cells; tile t owns
+// the contiguous new ids [tile_cell0[t], tile_cell0[t+1]).  Renumbering is a
+// storage permutation only: every edge keeps the reference's left/right
+// orientation and every cell keeps its local edge order, so the arithmetic is
+// unchanged.
+// ---------------------------------------------------------------------------
+struct TiledMesh {
+  int ncell = 0, ntile = 0, tile_cells = 0;
+  std::vector<int> new_to_old, old_to_new;  // [ncell]
+  // per cell, new numbering
+  std::vector<double> gvx, gvy;   // [3*ncell] local edge vectors (reference gradient form)
+  std::vector<double> inv_area;   // [ncell] 1.0 / area
+  std::vector<double> area;       // [ncell]
+  std::vector<int> nbr;           // [3*ncell] new id, or -1 far / -2 wall
+  std::vector<int> clog;          // [ncell]
+  std::vector<int> slot;          // [3*ncell] 2*(tile edge index) + side (0 left, 1 right)
+  std::vector<int> nloc;          // [3*ncell] tile-local index of interior neighbour, -1 otherwise
+  // per tile
+  std::vector<int> tile_cell0;    // [ntile+1]
+  std::vector<int> tile_halo0;    // [ntile+1] into halo
+  std::vector<int> halo;          // new ids of first-ring cells outside their tile
+  std::vector<int> tile_edge0;    // [ntile+1] into the tile edge records
+  // tile edge records
+  std::vector<int> e_l, e_r;      // tile-local left / right (right -1 far, -2 wall)
+  std::vector<double> e_nx, e_ny, e_vn, e_vx, e_vy, e_rlx, e_rly, e_rrx, e_rry;
+  int max_local = 0, max_edges = 0;  // largest tile-local cell and edge counts
+  // wall edges in wall-id order (forces)
+  std::vector<int> w_cell;        // new id of the wall edge's cell
+  std::vector<double> w_vx, w_vy, w_vn;
+};
+
+TiledMesh build_tiled(const Topology& topo, int tile_cells);
+
+// ---------------------------------------------------------------------------
+// Device records and kernel arguments.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct alignas(16) CellGeom {   // per cell: gradient geometry and area
+  R vx[3], vy[3], inv_area, area;
+};
+
+template <typename R>
+struct EdgeGeom;                // per tile edge (see fvm_tiled.h)
+
+struct TileDesc {               // one (case, mesh tile) pair of a batch
+  int icase;
+  int geo0;                     // offset of the case's mesh in the per-cell geometry arrays
+  int st0;                      // offset of the case in the per-cell state arrays
+  int cell0, ncell;             // mesh-local new ids owned by the tile
+  int halo0, nhalo;             // range in the halo array (global)
+  int edge0, nedge;             // range in the edge record array (global)
+};
+
+struct CaseCtl {                // per-case run control, device resident
+  int active;                   // 1 while step < max_steps and resd > epsl
+  int step;                     // steps done
+  int status;                   // fvm_status of the case
+  int tile0, ntile;             // the case's tiles in the batch tile list
+  int max_steps;
+  double epsl;
+  double epsm;                  // Resd normaliser (drou of step 1)
+  double resd;                  // last Resd
+};
+
+}  // namespace fvm

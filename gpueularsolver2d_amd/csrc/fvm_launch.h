@@ -1,0 +1,32 @@
+// Launch wrappers of the tiled kernels, instantiated once per working
+// precision (fvm_tiled64.hip, fvm_tiled32.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fvm_tiled.h"
+
+namespace fvm {
+
+template <typename R>
+hipError_t launch_flux_tile(const TiledArgs<R>& a, int threads, size_t lds, hipStream_t s);
+template <typename R>
+hipError_t launch_update_tile(const TiledArgs<R>& a, int threads, size_t lds, hipStream_t s);
+template <typename R>
+hipError_t launch_global_dt_min(const CaseCtl* ctl, const R* tile_min, R* dt_global, int ncase,
+                                hipStream_t s);
+// Largest dynamic LDS the two tile kernels may use (bytes).
+template <typename R>
+hipError_t set_tile_lds_limit(size_t bytes);
+
+template <typename R>
+inline size_t flux_tile_lds(int max_local, int max_edges) {
+  return (size_t)max_local * (sizeof(SVec4<R>) + 16 * sizeof(R)) +
+         (size_t)max_edges * (sizeof(SVec4<R>) + sizeof(R));
+}
+template <typename R>
+inline size_t update_tile_lds(int max_local) {
+  return (size_t)max_local * sizeof(SVec4<R>);
+}
+
+}  // namespace fvm

@@ -1,0 +1,346 @@
+// Tiled kernels of the explicit finite-volume Euler path: two launches per
+// Runge-Kutta stage, each covering every tile of every case of a batch.
+//
+// Kernel A (flux_tile): one workgroup per tile of cells.
+//   1. Green-Gauss gradient and neighbourhood bounds of the tile's cells and
+//      of its first-ring halo, into LDS (kernel_Gradient, gpu_solver.cu:
+//      225-383, with the ghost states of kernel_Boundary :137-223 computed
+//      in place from the boundary cell's own state).
+//   2. Roe flux and spectral radius of every edge touching the tile, into
+//      LDS edge slots (kernel_Flux2 :651-764, kernel_deltT :385-454).  Edges
+//      cut by the tile border are evaluated by both tiles from identical
+//      inputs, so both see bitwise the same flux.
+//   3. Ordered gather of each cell's three slots and the local time step
+//      (kernel_Flux_e2c :768-878); writes R and dt.
+// Kernel B (update_tile): one workgroup per tile.
+//   1. First smoothing sweep for the tile and its halo into LDS
+//      (kernel_Smooth1 :1635-1673).
+//   2. Second sweep and selection (kernel_Smooth2 :1675-1720), then the RK
+//      update W = W0 - a dt R / A (kernel_RKStep :938-966); in the last stage
+//      the tile's FP64 partial sum of (rho - rho0)^2 (kernel_drou :968-1020).
+// finalize_step: per case, the FP64 residual and the stop test of the
+// iteration loop (gpu_solver.cu:2403-2433).
+//
+// Determinism: every sum has a fixed order; there are no atomics.
+#pragma once
+
+#include "fvm_internal.h"
+#include "fvm_numerics.h"
+
+namespace fvm {
+
+template <typename R>
+struct alignas(16) SVec4 {
+  R d, du, dv, de;
+};
+
+template <typename R>
+struct alignas(16) EdgeRec {
+  R nx, ny, vn, vx, vy, rlx, rly, rrx, rry;
+  int l, r;  // tile-local cell indices; r = -1 far field, -2 wall
+};
+
+template <typename R>
+struct TiledArgs {
+  const TileDesc* tiles;
+  int ntiles;
+  CaseCtl* ctl;
+  const FlowT<R>* flow;       // per case
+  const CellGeom<R>* cg;      // per mesh cell (all meshes concatenated)
+  const int4* nbr;            // per mesh cell: neighbours (new ids, -1 far, -2 wall), .w = CLog
+  const int4* slot;           // per mesh cell: 2*slot + side for the three local edges
+  const int4* nloc;           // per mesh cell: tile-local index of interior neighbours or -1
+  const int* halo;
+  const EdgeRec<R>* edges;
+  const SVec4<R>* w_in;       // state read by kernel A
+  SVec4<R>* res;              // per case cell
+  R* dt;                      // per case cell (local dt, or A*CFL/T in global mode)
+  R* tile_min;                // per tile minimum of A*CFL/T (global dt mode)
+  const R* dt_global;         // per case (global dt mode)
+  const SVec4<R>* w0;         // step-start state
+  SVec4<R>* w_out;            // kernel B output
+  double* partial;            // per tile sum of (rho - rho0)^2
+  R rk;                       // RK stage coefficient
+  int last_stage;             // 1 in the fourth stage
+  int order, smoothing, global_dt;
+  int max_local, max_edges;
+};
+
+__device__ __forceinline__ int tile_of_block(int bid, int ntiles) {
+  // Consecutive workgroup ids go to different XCDs (round robin over 8).  Give
+  // each XCD a contiguous range of tiles so that neighbouring tiles, which
+  // share halo cells, hit the same L2.
+  const int x = bid & 7, k = bid >> 3;
+  const int q = ntiles >> 3, r = ntiles & 7;
+  return x * q + (x < r ? x : r) + k;
+}
+
+template <typename R>
+__device__ __forceinline__ Vec4<R> ld(const SVec4<R>& s) {
+  return Vec4<R>{s.d, s.du, s.dv, s.de};
+}
+template <typename R>
+__device__ __forceinline__ SVec4<R> st(const Vec4<R>& v) {
+  return SVec4<R>{v.d, v.du, v.dv, v.de};
+}
+
+template <typename R>
+__device__ __forceinline__ FlowT<R> load_flow(const FlowT<R>* f) {
+  return *f;
+}
+
+// --------------------------------------------------------------------------
+// Kernel A: gradient + flux + gather for one tile.
+// LDS: W[nl] | grad[8][nl] | bounds[8][nl] | F[ne] | SR[ne]
+// --------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) flux_tile(TiledArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int t = tile_of_block(blockIdx.x, a.ntiles);
+  const TileDesc d = a.tiles[t];
+  if (!a.ctl[d.icase].active) return;
+  const FlowT<R> f = load_flow(a.flow + d.icase);
+  const int nt = d.ncell, nl = d.ncell + d.nhalo, ne = d.nedge;
+  const int ML = a.max_local, ME = a.max_edges;
+  SVec4<R>* sW = reinterpret_cast<SVec4<R>*>(smem);
+  R* sG = reinterpret_cast<R*>(sW + ML);  // [8][ML]: d/dx 4 comps, d/dy 4 comps
+  R* sB = sG + 8 * ML;                    // [8][ML]: max 4 comps, min 4 comps
+  SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
+  R* sS = reinterpret_cast<R*>(sF + ME);
+  const SVec4<R>* W = a.w_in + d.st0;
+
+  // 1. gradient and bounds of tile + halo cells
+  for (int j = threadIdx.x; j < nl; j += blockDim.x) {
+    const int g = j < nt ? d.cell0 + j : a.halo[d.halo0 + j - nt];
+    const Vec4<R> Wi = ld(W[g]);
+    sW[j] = st(Wi);
+    if (a.order >= 2) {
+      const CellGeom<R> cg = a.cg[d.geo0 + g];
+      const int4 nb = a.nbr[d.geo0 + g];
+      const int nbk[3] = {nb.x, nb.y, nb.z};
+      R gx[4] = {R(0.0), R(0.0), R(0.0), R(0.0)}, gy[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
+      R mx[4], mn[4];
+      const R ui[4] = {Wi.d, Wi.du, Wi.dv, Wi.de};
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        Vec4<R> c;
+        if (nbk[k] >= 0)
+          c = ld(W[nbk[k]]);
+        else
+          c = boundary_ghost<R>(-nbk[k], Wi, cg.vx[k], cg.vy[k], f);
+        const R cc[4] = {c.d, c.du, c.dv, c.de};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          if (k == 0) {
+            mx[q] = cc[q];
+            mn[q] = cc[q];
+          } else {
+            if (cc[q] < mn[q]) mn[q] = cc[q];
+            if (cc[q] > mx[q]) mx[q] = cc[q];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) gx[q] += R(0.5) * (ui[q] + cc[q]) * cg.vy[k];
+#pragma unroll
+        for (int q = 0; q < 4; q++) gy[q] += R(0.5) * (ui[q] + cc[q]) * cg.vx[k];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        sG[q * ML + j] = gx[q] * cg.inv_area;
+        sG[(4 + q) * ML + j] = gy[q] * cg.inv_area;
+        sB[q * ML + j] = fmax(ui[q], mx[q]);
+        sB[(4 + q) * ML + j] = fmin(ui[q], mn[q]);
+      }
+    }
+  }
+  __syncthreads();
+
+  // 2. flux and spectral radius of the tile's edges
+  for (int s = threadIdx.x; s < ne; s += blockDim.x) {
+    const EdgeRec<R> e = a.edges[d.edge0 + s];
+    const Vec4<R> WL = ld(sW[e.l]);
+    Vec4<R> L = WL, Rr, Rraw;
+    if (e.r >= 0) {
+      Rraw = ld(sW[e.r]);
+      Rr = Rraw;
+      if (a.order >= 2) {
+        // limited reconstruction, kernel_LBJinterpolate gpu_solver.cu:496-549
+        const int l = e.l, r = e.r;
+        R dl[4], dr[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          dl[q] = sG[q * ML + l] * e.rlx + sG[(4 + q) * ML + l] * e.rly;
+          dr[q] = sG[q * ML + r] * e.rrx + sG[(4 + q) * ML + r] * e.rry;
+        }
+        const R ul[4] = {WL.d, WL.du, WL.dv, WL.de};
+        const R ur[4] = {Rraw.d, Rraw.du, Rraw.dv, Rraw.de};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          dl[q] = limit(dl[q], ul[q], sB[q * ML + l], sB[(4 + q) * ML + l]);
+          dr[q] = limit(dr[q], ur[q], sB[q * ML + r], sB[(4 + q) * ML + r]);
+        }
+        L = Vec4<R>{ul[0] + dl[0], ul[1] + dl[1], ul[2] + dl[2], ul[3] + dl[3]};
+        Rr = Vec4<R>{ur[0] + dr[0], ur[1] + dr[1], ur[2] + dr[2], ur[3] + dr[3]};
+      }
+    } else if (e.r == -1) {
+      Rraw = far_ghost(WL, e.vx, e.vy, f);
+      Rr = Rraw;
+    } else {
+      Rraw = wall_ghost(WL, e.vx, e.vy, e.vn);
+      Rr = Rraw;
+    }
+    Vec4<R> F = roe_flux(L, Rr, e.nx, e.ny, f.gamma);
+    F.d *= e.vn;
+    F.du *= e.vn;
+    F.dv *= e.vn;
+    F.de *= e.vn;
+    sF[s] = st(F);
+    sS[s] = spectral_radius(WL, Rraw, e.vx, e.vy, f.gamma);
+  }
+  __syncthreads();
+
+  // 3. ordered gather and time step
+  R tmin = R(3.0e38);
+  for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+    const int g = d.cell0 + i;
+    const int4 sl = a.slot[d.geo0 + g];
+    const int slk[3] = {sl.x, sl.y, sl.z};
+    R T = 0;
+    R r0 = R(0.0), r1 = R(0.0), r2 = R(0.0), r3 = R(0.0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int s = slk[k] >> 1;
+      const SVec4<R> F = sF[s];
+      T += sS[s];
+      if ((slk[k] & 1) == 0) {
+        r0 += F.d; r1 += F.du; r2 += F.dv; r3 += F.de;
+      } else {
+        r0 -= F.d; r1 -= F.du; r2 -= F.dv; r3 -= F.de;
+      }
+    }
+    a.res[d.st0 + g] = SVec4<R>{r0, r1, r2, r3};
+    const R area = a.cg[d.geo0 + g].area;
+    if (a.global_dt) {
+      const R v = area * f.cfl / T;
+      a.dt[d.st0 + g] = v;
+      tmin = fmin(tmin, v);
+    } else {
+      a.dt[d.st0 + g] = fmin(area * f.cfl / T, R(0.1));
+    }
+  }
+  if (a.global_dt) {
+    __shared__ R red[256];
+    red[threadIdx.x] = tmin;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      R m = red[0];
+      for (int k = 1; k < (int)blockDim.x; k++) m = fmin(m, red[k]);
+      a.tile_min[t] = m;
+    }
+  }
+}
+
+// Per-case minimum of the tile minima (global time step mode).
+template <typename R>
+__global__ void global_dt_min(const CaseCtl* ctl, const R* tile_min, R* dt_global, int ncase) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncase || !ctl[c].active) return;
+  R m = R(9999999.0);
+  for (int k = 0; k < ctl[c].ntile; k++) m = fmin(m, tile_min[ctl[c].tile0 + k]);
+  dt_global[c] = m;
+}
+
+// --------------------------------------------------------------------------
+// Kernel B: smoothing + RK update for one tile.  LDS: R1[nl]
+// --------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) update_tile(TiledArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int t = tile_of_block(blockIdx.x, a.ntiles);
+  const TileDesc d = a.tiles[t];
+  if (!a.ctl[d.icase].active) return;
+  const FlowT<R> f = load_flow(a.flow + d.icase);
+  const int nt = d.ncell, nl = d.ncell + d.nhalo;
+  SVec4<R>* sR1 = reinterpret_cast<SVec4<R>*>(smem);
+  const SVec4<R>* Rs = a.res + d.st0;
+  const R eps = f.eps;
+  if (a.smoothing) {
+    for (int j = threadIdx.x; j < nl; j += blockDim.x) {
+      const int g = j < nt ? d.cell0 + j : a.halo[d.halo0 + j - nt];
+      const int4 nb = a.nbr[d.geo0 + g];
+      const int nbk[3] = {nb.x, nb.y, nb.z};
+      R s0 = R(0.0), s1 = R(0.0), s2 = R(0.0), s3 = R(0.0);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        if (nbk[k] >= 0) {
+          const SVec4<R> v = Rs[nbk[k]];
+          s0 += v.d; s1 += v.du; s2 += v.dv; s3 += v.de;
+        }
+      }
+      const SVec4<R> ri = Rs[g];
+      const R den = R(1.0) + eps * R(nb.w);
+      sR1[j] = SVec4<R>{(s0 * eps + ri.d) / den, (s1 * eps + ri.du) / den,
+                        (s2 * eps + ri.dv) / den, (s3 * eps + ri.de) / den};
+    }
+    __syncthreads();
+  }
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+    const int g = d.cell0 + i;
+    SVec4<R> ri = Rs[g];
+    if (a.smoothing) {
+      const int4 nl4 = a.nloc[d.geo0 + g];
+      const int lk[3] = {nl4.x, nl4.y, nl4.z};
+      R s0 = R(0.0), s1 = R(0.0), s2 = R(0.0), s3 = R(0.0);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        if (lk[k] >= 0) {
+          const SVec4<R> v = sR1[lk[k]];
+          s0 += v.d; s1 += v.du; s2 += v.dv; s3 += v.de;
+        }
+      }
+      const R den = R(1.0) + eps * R(a.nbr[d.geo0 + g].w);
+      const R q0 = (s0 * eps + ri.d) / den, q1 = (s1 * eps + ri.du) / den;
+      const R q2 = (s2 * eps + ri.dv) / den, q3 = (s3 * eps + ri.de) / den;
+      if (fabs(ri.d) > fabs(q0)) ri.d = q0;
+      if (fabs(ri.du) > fabs(q1)) ri.du = q1;
+      if (fabs(ri.dv) > fabs(q2)) ri.dv = q2;
+      if (fabs(ri.de) > fabs(q3)) ri.de = q3;
+    }
+    const R dtc = a.global_dt ? a.dt_global[d.icase] : a.dt[d.st0 + g];
+    const R rk = a.rk * dtc;
+    const R area = a.cg[d.geo0 + g].area;
+    const SVec4<R> w0 = a.w0[d.st0 + g];
+    SVec4<R> wn;
+    wn.d = w0.d - rk * ri.d / area;
+    wn.du = w0.du - rk * ri.du / area;
+    wn.dv = w0.dv - rk * ri.dv / area;
+    wn.de = w0.de - rk * ri.de / area;
+    a.w_out[d.st0 + g] = wn;
+    if (a.last_stage) {
+      const double df = (double)wn.d - (double)w0.d;
+      acc += df * df;
+    }
+  }
+  if (a.last_stage) {
+    // fixed-order reduction: wave shuffles, then waves in order
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    __shared__ double wsum[16];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) wsum[wv] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int k = 0; k < (int)(blockDim.x + 63) / 64; k++) s += wsum[k];
+      a.partial[t] = s;
+    }
+  }
+}
+
+// Per case: sum of the tile partials (fixed order, compensated), the
+// normalised residual Resd = sqrt((drou + 1e-32) / drou_step1) and the stop
+// test step < max_steps && Resd > epsl (gpu_solver.cu:2200, :2428-2433).
+__global__ void finalize_step(CaseCtl* ctl, const double* partial, double* hist, int hist_stride,
+                              int ncase);
+
+}  // namespace fvm

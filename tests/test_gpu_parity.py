@@ -1,0 +1,108 @@
+"""GPU parity: the HIP solver through the C ABI against the FP64 CPU oracle.
+
+FP64 paths (plain and tiled kernels) must reproduce the oracle bit for bit:
+the arithmetic sequence is the same, every operation is correctly rounded and
+nothing is contracted, so any difference is a bug (and in the chaotic
+smoothing regime a 1-ulp difference grows ~10x per 30 steps).  The FP32 path
+is held to explicit tolerances.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from gpueularsolver2d_amd import Solver, naca0012_mesh, read_dat
+from oracle_ctypes import OracleMesh, OracleSolver
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rae3k():
+    return read_dat(os.path.join(GOLDEN, "2822-3k.dat.gz"))
+
+
+@pytest.fixture(scope="module")
+def naca3k():
+    return naca0012_mesh(3000, seed=1)
+
+
+def oracle_run(mesh, steps, mach, alpha, cfl, epsl=-1.0, **kw):
+    om = OracleMesh(mesh.x, mesh.y, mesh.tri)
+    s = OracleSolver(om, mach, alpha, cfl=cfl, **kw)
+    hist, _ = s.run(steps, epsl)
+    return s, hist
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+CASES = [
+    # (mach, alpha, cfl, order, smoothing, global_dt)
+    (0.729, 2.31, 1.0, 2, True, False),   # the reference's own RAE 2822 run
+    (0.729, 2.31, 1.0, 2, False, False),
+    (0.729, 2.31, 1.0, 2, True, True),
+    (0.5, 0.0, 1.5, 1, False, False),
+]
+
+
+@pytest.mark.parametrize("kernels", ["plain", "tiled"])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"M{c[0]}-a{c[1]}-o{c[3]}-s{int(c[4])}-g{int(c[5])}")
+def test_fp64_bitwise_vs_oracle(rae3k, kernels, case):
+    mach, alpha, cfl, order, smooth, gdt = case
+    steps = 40
+    orc, hist_o = oracle_run(rae3k, steps, mach, alpha, cfl, order=order, smoothing=smooth,
+                             global_dt=gdt)
+    s = Solver(rae3k, mach, alpha, cfl=cfl, order=order, smoothing=smooth, global_dt=gdt,
+               precision="fp64", kernels=kernels)
+    hist, nsteps, st = s.run(steps)
+    assert st == 0 and nsteps == steps
+    W = s.state()
+    Wo = orc.state()
+    assert np.array_equal(W, Wo), f"max |dW| = {np.abs(W - Wo).max():.3e}"
+    # residual: same arithmetic except the order of the FP64 partial sums
+    np.testing.assert_allclose(hist[1:], hist_o[1:steps + 1], rtol=1e-12, atol=0)
+    cl, cd = s.forces()
+    clo, cdo = orc.forces()
+    assert abs(cl - clo) <= 1e-12 * max(1.0, abs(clo))
+    assert abs(cd - cdo) <= 1e-12 * max(1.0, abs(cdo))
+
+
+def test_tiled_fp64_matches_oracle_naca(naca3k):
+    steps = 60
+    orc, hist_o = oracle_run(naca3k, steps, 0.8, 1.25, 1.5, order=2, smoothing=True)
+    for tile in (32, 160):
+        s = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp64", kernels="tiled", tile_cells=tile)
+        hist, n, st = s.run(steps)
+        assert st == 0 and n == steps
+        assert np.array_equal(s.state(), orc.state())
+
+
+def test_fp32_close_to_oracle(naca3k):
+    steps = 200
+    orc, hist_o = oracle_run(naca3k, steps, 0.8, 1.25, 1.5, order=2, smoothing=True)
+    s = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp32", kernels="tiled")
+    hist, n, st = s.run(steps)
+    assert st == 0 and n == steps
+    W, Wo = s.state(), orc.state()
+    err = rel_l2(W[:, 0], Wo[:, 0])
+    assert err < 1e-4, err
+    cl, cd = s.forces()
+    clo, cdo = orc.forces()
+    assert abs(cl - clo) < 1e-3, (cl, clo)
+    assert abs(cd - cdo) < 1e-3, (cd, cdo)
+
+
+def test_epsl_stop_matches_reference_loop(rae3k):
+    # while (step < MAXSTEP && Resd[step] > EPSL): stops at the first step
+    # whose residual is <= EPSL (gpu_solver.cu:2200)
+    orc, hist_o = oracle_run(rae3k, 300, 0.729, 2.31, 1.0, epsl=0.3, order=2, smoothing=True)
+    stop = len(hist_o) - 1
+    assert stop < 300
+    s = Solver(rae3k, 0.729, 2.31, cfl=1.0, precision="fp64", steps_per_graph=7)
+    hist, n, st = s.run(300, epsl=0.3)
+    assert n == stop
+    assert np.array_equal(s.state(), orc.state())
