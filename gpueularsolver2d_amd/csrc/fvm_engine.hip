@@ -1133,6 +1133,13 @@ fvm_status fvm_batch_get_config(fvm_batch* b, int32_t config[4]) {
   });
 }
 
+fvm_status fvm_get_config(fvm_solver* s, int32_t config[4]) {
+  return guard([&] {
+    if (!s || !config) throw Error{FVM_ERR_ARGUMENT, "NULL argument"};
+    s->b->config(config);
+  });
+}
+
 fvm_status fvm_batch_kernel_times(fvm_batch* b, int32_t reps, double ms[2]) {
   return guard([&] {
     if (!b || !ms) throw Error{FVM_ERR_ARGUMENT, "NULL argument"};

@@ -32,8 +32,8 @@ EXPORTS = ("fvm_default_opts", "fvm_status_string", "fvm_last_error", "fvm_mesh_
            "fvm_create", "fvm_run", "fvm_get_state", "fvm_set_state", "fvm_get_forces",
            "fvm_get_timing", "fvm_destroy", "fvm_batch_create", "fvm_batch_run",
            "fvm_batch_get_state", "fvm_batch_get_forces", "fvm_batch_get_timing",
-           "fvm_batch_get_config", "fvm_batch_kernel_times", "fvm_kernel_times",
-           "fvm_batch_destroy")
+           "fvm_batch_get_config", "fvm_get_config", "fvm_batch_kernel_times",
+           "fvm_kernel_times", "fvm_batch_destroy")
 
 
 class FvmError(RuntimeError):
@@ -98,6 +98,7 @@ def load_library(path=LIB_PATH):
     L.fvm_batch_get_forces.argtypes = [C.c_void_p, _dp, _dp]
     L.fvm_batch_get_timing.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int64)]
     L.fvm_batch_get_config.argtypes = [C.c_void_p, _ip]
+    L.fvm_get_config.argtypes = [C.c_void_p, _ip]
     L.fvm_batch_kernel_times.argtypes = [C.c_void_p, C.c_int32, _dp]
     L.fvm_kernel_times.argtypes = [C.c_void_p, C.c_int32, _dp]
     L.fvm_batch_destroy.argtypes = [C.c_void_p]
@@ -105,7 +106,8 @@ def load_library(path=LIB_PATH):
     for name in ("fvm_mesh_topology", "fvm_create", "fvm_run", "fvm_get_state", "fvm_set_state",
                  "fvm_get_forces", "fvm_get_timing", "fvm_batch_create", "fvm_batch_run",
                  "fvm_batch_get_state", "fvm_batch_get_forces", "fvm_batch_get_timing",
-                 "fvm_batch_get_config", "fvm_batch_kernel_times", "fvm_kernel_times"):
+                 "fvm_batch_get_config", "fvm_get_config", "fvm_batch_kernel_times",
+                 "fvm_kernel_times"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -228,6 +230,12 @@ class Solver:
         ms, csu = C.c_double(), C.c_int64()
         _check(load_library().fvm_get_timing(self.h, C.byref(ms), C.byref(csu)))
         return ms.value, csu.value
+
+    def config(self):
+        cfg = np.zeros(4, np.int32)
+        _check(load_library().fvm_get_config(self.h, _iptr(cfg)))
+        return dict(tile_cells=int(cfg[0]), threads=int(cfg[1]), steps_per_graph=int(cfg[2]),
+                    tiles=int(cfg[3]))
 
     def kernel_times(self, reps=20):
         """Average ms per launch of (flux_tile, update_tile)."""

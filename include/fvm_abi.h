@@ -170,6 +170,7 @@ fvm_status fvm_batch_kernel_times(fvm_batch *b, int32_t reps, double ms[2]);
 fvm_status fvm_kernel_times(fvm_solver *s, int32_t reps, double ms[2]);
 /* Launch shape in use: {tile_cells, threads, steps_per_graph, total tiles}. */
 fvm_status fvm_batch_get_config(fvm_batch *b, int32_t config[4]);
+fvm_status fvm_get_config(fvm_solver *s, int32_t config[4]);
 void fvm_batch_destroy(fvm_batch *b);
 
 #ifdef __cplusplus
