@@ -1,0 +1,52 @@
+"""The synthetic NACA 0012 mesh generator and the reference's mesh format."""
+import numpy as np
+import pytest
+
+from gpueularsolver2d_amd.meshgen import (N_FAR, N_SURFACE, naca0012_mesh, naca0012_surface,
+                                          read_dat, write_dat)
+from oracle_ctypes import OracleMesh
+
+
+def test_surface_is_symmetric_and_closed_form():
+    x, y = naca0012_surface()
+    assert len(x) == N_SURFACE
+    half = N_SURFACE // 2
+    np.testing.assert_array_equal(x[:half], x[half:][::-1])
+    np.testing.assert_array_equal(y[:half], -y[half:][::-1])
+    # maximum thickness of NACA 0012 is 0.12 chord near x = 0.3
+    assert 0.118 < 2 * y.max() < 0.1201
+    assert 0.25 < x[np.argmax(y)] < 0.35
+
+
+@pytest.mark.parametrize("target,seed", [(3000, 1), (10000, 12)])
+def test_mesh_valid_and_deterministic(target, seed):
+    m = naca0012_mesh(target, seed)
+    assert abs(m.ncell - target) < 0.05 * target
+    om = OracleMesh(m.x, m.y, m.tri)
+    assert (om.area[1:] > 0).all()
+    assert om.nwall == N_SURFACE and om.nfar == N_FAR
+    # Euler: a triangulated annulus has T = 2V - B (B boundary vertices)
+    assert m.ncell == 2 * m.nnode - (N_SURFACE + N_FAR)
+    m2 = naca0012_mesh(target, seed)
+    np.testing.assert_array_equal(m.x, m2.x)
+    np.testing.assert_array_equal(m.tri, m2.tri)
+    m3 = naca0012_mesh(target, seed + 1)
+    assert m3.nnode != m.nnode or not np.array_equal(m3.x, m.x)
+
+
+def test_dat_round_trip(tmp_path):
+    m = naca0012_mesh(3000, 1)
+    for name in ("m.dat", "m.dat.gz"):
+        p = tmp_path / name
+        write_dat(m, p)
+        r = read_dat(p)
+        np.testing.assert_array_equal(r.x, m.x)
+        np.testing.assert_array_equal(r.y, m.y)
+        np.testing.assert_array_equal(r.tri, m.tri)
+
+
+def test_truncated_file_rejected(tmp_path):
+    p = tmp_path / "bad.dat"
+    p.write_text("10 5\n0 1 2\n")
+    with pytest.raises(ValueError):
+        read_dat(p)
