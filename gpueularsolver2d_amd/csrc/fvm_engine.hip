@@ -39,55 +39,15 @@ static const double kRK[4] = {0.0833, 0.2069, 0.4265, 1.0};  // gpu_fvm.cu:10
 // ---------------------------------------------------------------------------
 // finalize_step: per case residual and stop test
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void neumaier(double& s, double& c, double v) {
-  const double t = s + v;
-  if (fabs(s) >= fabs(v))
-    c += (s - t) + v;
-  else
-    c += (v - t) + s;
-  s = t;
-}
-
+// Closes the time step of every active case after the last stage (one
+// workgroup of 64 per case).  A separate launch: the kernel boundary orders
+// the tiles' partial sums before the close without a device-scope fence,
+// which on MI355X would write back a whole L2 per workgroup.
 __global__ void __launch_bounds__(64) finalize_kernel(CaseCtl* ctl, const double* partial,
-                                                      double* hist, int hist_stride, int sequential) {
+                                                      double* hist, int hist_stride) {
   CaseCtl& c = ctl[blockIdx.x];
   if (!c.active) return;
-  __shared__ double part[64];
-  double drou;
-  if (sequential) {
-    // the reference's host loop over block partials, gpu_solver.cu:2410-2414
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int k = 0; k < c.ntile; k++) s += partial[c.tile0 + k];
-      part[0] = s;
-    }
-    __syncthreads();
-    drou = part[0];
-  } else {
-    const int n = c.ntile, per = (n + 63) / 64;
-    const int b = threadIdx.x * per, e = min(n, b + per);
-    double s = 0.0, cc = 0.0;
-    for (int k = b; k < e; k++) neumaier(s, cc, partial[c.tile0 + k]);
-    part[threadIdx.x] = s + cc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s2 = 0.0, c2 = 0.0;
-      for (int k = 0; k < 64; k++) neumaier(s2, c2, part[k]);
-      part[0] = s2 + c2;
-    }
-    __syncthreads();
-    drou = part[0];
-  }
-  if (threadIdx.x == 0) {
-    const int step = c.step + 1;
-    c.step = step;
-    if (step == 1) c.epsm = drou;
-    const double resd = sqrt((drou + 1.0E-32) / c.epsm);
-    c.resd = resd;
-    if (step < hist_stride) hist[(size_t)blockIdx.x * hist_stride + step] = resd;
-    if (!isfinite(resd)) c.status = FVM_ERR_NAN;
-    if (!(step < c.max_steps && resd > c.epsl)) c.active = 0;
-  }
+  close_step(c, partial, hist, hist_stride, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -300,8 +260,8 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
     throw Error{FVM_ERR_ARGUMENT, "opts.kernels must be FVM_KERNELS_TILED or FVM_KERNELS_PLAIN"};
   if (o.kernels == FVM_KERNELS_PLAIN && o.precision != FVM_FP64)
     throw Error{FVM_ERR_ARGUMENT, "the plain kernels run in FP64 only"};
-  if (o.threads != 0 && (o.threads < 64 || o.threads > 256 || o.threads % 64 != 0))
-    throw Error{FVM_ERR_ARGUMENT, "opts.threads must be 0 or one of 64, 128, 192, 256"};
+  if (o.threads != 0 && (o.threads < 64 || o.threads > 512 || o.threads % 64 != 0))
+    throw Error{FVM_ERR_ARGUMENT, "opts.threads must be 0 or a multiple of 64 up to 512"};
   for (int i = 0; i < ncase; i++) {
     if (case_mesh[i] < 0 || case_mesh[i] >= nmesh)
       throw Error{FVM_ERR_ARGUMENT, "case_mesh[" + std::to_string(i) + "] out of range"};
@@ -319,7 +279,11 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
     cases_.push_back(ci);
   }
   if (o.kernels == FVM_KERNELS_TILED) {
-    tile_cells_ = o.tile_cells > 0 ? o.tile_cells : (o.precision == FVM_FP64 ? 160 : 256);
+    // defaults from the launch-shape sweep on MI355X (autotune.py): 128-cell
+    // tiles with 256 threads give one pass per phase for both precisions
+    tile_cells_ = o.tile_cells > 0 ? o.tile_cells : 128;
+    if (tile_cells_ > 8192)
+      throw Error{FVM_ERR_ARGUMENT, "opts.tile_cells must be at most 8192"};
     threads_ = o.threads > 0 ? o.threads : 256;
     for (int m = 0; m < nmesh; m++) tiled_.push_back(build_tiled(topo_[m], tile_cells_));
   }
@@ -480,18 +444,9 @@ void Batch::setup_tiled() {
     }
     std::copy(T.halo.begin(), T.halo.end(), hhalo.begin() + halo0[m]);
     for (size_t s = 0; s < T.e_l.size(); s++) {
-      EdgeRec<R>& e = hedge[edge0[m] + s];
-      e.nx = (R)T.e_nx[s];
-      e.ny = (R)T.e_ny[s];
-      e.vn = (R)T.e_vn[s];
-      e.vx = (R)T.e_vx[s];
-      e.vy = (R)T.e_vy[s];
-      e.rlx = (R)T.e_rlx[s];
-      e.rly = (R)T.e_rly[s];
-      e.rrx = (R)T.e_rrx[s];
-      e.rry = (R)T.e_rry[s];
-      e.l = T.e_l[s];
-      e.r = T.e_r[s];
+      hedge[edge0[m] + s] = make_edge_rec(T.e_nx[s], T.e_ny[s], T.e_vn[s], T.e_vx[s], T.e_vy[s],
+                                          T.e_rlx[s], T.e_rly[s], T.e_rrx[s], T.e_rry[s], T.e_l[s],
+                                          T.e_r[s], (R*)nullptr);
     }
     for (size_t w = 0; w < T.w_cell.size(); w++) {
       hwcell[wall0[m] + w] = T.w_cell[w];
@@ -738,7 +693,7 @@ void Batch::enqueue_tiled_steps(int nsteps) {
       HIP_OK(launch_update_tile<R>(a, threads_, lds_b_, stream_));
     }
     hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(64), 0, stream_, d_ctl_,
-                       d_partial_, d_hist_, hist_stride_, 0);
+                       d_partial_, d_hist_, hist_stride_);
     HIP_OK(hipGetLastError());
   }
 }
@@ -794,7 +749,7 @@ void Batch::enqueue_plain_steps(int nsteps) {
                         stream_);
     HIP_OK(hipGetLastError());
     hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(64), 0, stream_, d_ctl_,
-                       d_partial_, d_hist_, hist_stride_, 1);
+                       d_partial_, d_hist_, hist_stride_);
     HIP_OK(hipGetLastError());
   }
 }

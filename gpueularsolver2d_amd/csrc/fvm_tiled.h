@@ -18,7 +18,7 @@
 //   2. Second sweep and selection (kernel_Smooth2 :1675-1720), then the RK
 //      update W = W0 - a dt R / A (kernel_RKStep :938-966); in the last stage
 //      the tile's FP64 partial sum of (rho - rho0)^2 (kernel_drou :968-1020).
-// finalize_step: per case, the FP64 residual and the stop test of the
+// finalize_kernel: per case, the FP64 residual and the stop test of the
 // iteration loop (gpu_solver.cu:2403-2433).
 //
 // Determinism: every sum has a fixed order; there are no atomics.
@@ -34,11 +34,52 @@ struct alignas(16) SVec4 {
   R d, du, dv, de;
 };
 
+// Per tile edge: unit normal (n = (vy, vx)/|v|, the reference's convention),
+// length, reconstruction offsets mid - centroid of both cells, and the
+// tile-local indices of the two cells (right: -1 far field, -2 wall).
 template <typename R>
-struct alignas(16) EdgeRec {
-  R nx, ny, vn, vx, vy, rlx, rly, rrx, rry;
-  int l, r;  // tile-local cell indices; r = -1 far field, -2 wall
+struct EdgeRec;
+
+// FP64: the edge vector itself is kept because the spectral radius of the
+// reference uses it directly (u vy + v vx, vx^2 + vy^2).
+template <>
+struct alignas(16) EdgeRec<double> {
+  double nx, ny, vn, vx, vy, rlx, rly, rrx, rry;
+  int l, r;
 };
+
+// FP32: 32 bytes; the edge vector is recovered as (ny vn, nx vn) and the two
+// local indices are packed as 16-bit halves.
+template <>
+struct alignas(16) EdgeRec<float> {
+  float nx, ny, vn, rlx, rly, rrx, rry;
+  int lr;  // low 16 bits: left (unsigned), high 16 bits: right (signed)
+};
+
+__host__ __device__ inline EdgeRec<double> make_edge_rec(double nx, double ny, double vn, double vx,
+                                                         double vy, double rlx, double rly,
+                                                         double rrx, double rry, int l, int r,
+                                                         double*) {
+  return EdgeRec<double>{nx, ny, vn, vx, vy, rlx, rly, rrx, rry, l, r};
+}
+__host__ __device__ inline EdgeRec<float> make_edge_rec(double nx, double ny, double vn, double, double,
+                                                        double rlx, double rly, double rrx,
+                                                        double rry, int l, int r, float*) {
+  return EdgeRec<float>{(float)nx, (float)ny, (float)vn, (float)rlx, (float)rly, (float)rrx,
+                        (float)rry, (int)(((unsigned)r << 16) | ((unsigned)l & 0xffffu))};
+}
+
+struct EdgeView {
+  int l, r;
+};
+__device__ __forceinline__ EdgeView edge_cells(const EdgeRec<double>& e) { return {e.l, e.r}; }
+__device__ __forceinline__ EdgeView edge_cells(const EdgeRec<float>& e) {
+  return {e.lr & 0xffff, e.lr >> 16};
+}
+__device__ __forceinline__ double edge_vx(const EdgeRec<double>& e) { return e.vx; }
+__device__ __forceinline__ double edge_vy(const EdgeRec<double>& e) { return e.vy; }
+__device__ __forceinline__ float edge_vx(const EdgeRec<float>& e) { return e.ny * e.vn; }
+__device__ __forceinline__ float edge_vy(const EdgeRec<float>& e) { return e.nx * e.vn; }
 
 template <typename R>
 struct TiledArgs {
@@ -84,9 +125,42 @@ __device__ __forceinline__ SVec4<R> st(const Vec4<R>& v) {
   return SVec4<R>{v.d, v.du, v.dv, v.de};
 }
 
-template <typename R>
-__device__ __forceinline__ FlowT<R> load_flow(const FlowT<R>* f) {
-  return *f;
+__device__ __forceinline__ void neumaier(double& s, double& c, double v) {
+  const double t = s + v;
+  if (fabs(s) >= fabs(v))
+    c += (s - t) + v;
+  else
+    c += (v - t) + s;
+  s = t;
+}
+
+// Closes one time step of case c from its per-tile (or per-block) partial
+// sums of (rho - rho0)^2: fixed-order compensated sum over 64 lanes, then the
+// normalised residual Resd = sqrt((drou + 1e-32) / drou_of_step_1), the
+// history entry and the stop test step < max_steps && Resd > epsl
+// (gpu_solver.cu:2200, :2403-2433).  Called by one full 64-lane wave.
+__device__ inline void close_step(CaseCtl& c, const double* partial, double* hist, int hist_stride,
+                                  int icase) {
+  const int lane = threadIdx.x & 63;
+  const int n = c.ntile, per = (n + 63) / 64;
+  const int b = lane * per, e = min(n, b + per);
+  double s = 0.0, cc = 0.0;
+  for (int k = b; k < e; k++) neumaier(s, cc, partial[c.tile0 + k]);
+  double v = s + cc;
+  // lane-ordered combination (lane 0 first), the same on every run
+  double tot = 0.0, ctot = 0.0;
+  for (int k = 0; k < 64; k++) neumaier(tot, ctot, __shfl(v, k, 64));
+  const double drou = tot + ctot;
+  if (lane == 0) {
+    const int step = c.step + 1;
+    c.step = step;
+    if (step == 1) c.epsm = drou;
+    const double resd = sqrt((drou + 1.0E-32) / c.epsm);
+    c.resd = resd;
+    if (step < hist_stride) hist[(size_t)icase * hist_stride + step] = resd;
+    if (!isfinite(resd)) c.status = FVM_ERR_NAN;
+    if (!(step < c.max_steps && resd > c.epsl)) c.active = 0;
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -94,12 +168,12 @@ __device__ __forceinline__ FlowT<R> load_flow(const FlowT<R>* f) {
 // LDS: W[nl] | grad[8][nl] | bounds[8][nl] | F[ne] | SR[ne]
 // --------------------------------------------------------------------------
 template <typename R>
-__global__ void __launch_bounds__(256) flux_tile(TiledArgs<R> a) {
+__global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int t = tile_of_block(blockIdx.x, a.ntiles);
   const TileDesc d = a.tiles[t];
   if (!a.ctl[d.icase].active) return;
-  const FlowT<R> f = load_flow(a.flow + d.icase);
+  const FlowT<R> f = a.flow[d.icase];
   const int nt = d.ncell, nl = d.ncell + d.nhalo, ne = d.nedge;
   const int ML = a.max_local, ME = a.max_edges;
   SVec4<R>* sW = reinterpret_cast<SVec4<R>*>(smem);
@@ -108,6 +182,18 @@ __global__ void __launch_bounds__(256) flux_tile(TiledArgs<R> a) {
   SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
   R* sS = reinterpret_cast<R*>(sF + ME);
   const SVec4<R>* W = a.w_in + d.st0;
+
+  // Prefetch the first edge record and gather slot of this thread; their
+  // latency then hides behind the gradient phase.
+  const int s0 = threadIdx.x;
+  EdgeRec<R> e0{};
+  if (s0 < ne) e0 = a.edges[d.edge0 + s0];
+  int4 sl0 = make_int4(0, 0, 0, 0);
+  R area0 = R(0);
+  if (s0 < nt) {
+    sl0 = a.slot[d.geo0 + d.cell0 + s0];
+    area0 = a.cg[d.geo0 + d.cell0 + s0].area;
+  }
 
   // 1. gradient and bounds of tile + halo cells
   for (int j = threadIdx.x; j < nl; j += blockDim.x) {
@@ -157,15 +243,17 @@ __global__ void __launch_bounds__(256) flux_tile(TiledArgs<R> a) {
 
   // 2. flux and spectral radius of the tile's edges
   for (int s = threadIdx.x; s < ne; s += blockDim.x) {
-    const EdgeRec<R> e = a.edges[d.edge0 + s];
-    const Vec4<R> WL = ld(sW[e.l]);
+    const EdgeRec<R> e = (s == s0) ? e0 : a.edges[d.edge0 + s];
+    const EdgeView ev = edge_cells(e);
+    const R vx = edge_vx(e), vy = edge_vy(e);
+    const Vec4<R> WL = ld(sW[ev.l]);
     Vec4<R> L = WL, Rr, Rraw;
-    if (e.r >= 0) {
-      Rraw = ld(sW[e.r]);
+    if (ev.r >= 0) {
+      Rraw = ld(sW[ev.r]);
       Rr = Rraw;
       if (a.order >= 2) {
         // limited reconstruction, kernel_LBJinterpolate gpu_solver.cu:496-549
-        const int l = e.l, r = e.r;
+        const int l = ev.l, r = ev.r;
         R dl[4], dr[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -182,11 +270,11 @@ __global__ void __launch_bounds__(256) flux_tile(TiledArgs<R> a) {
         L = Vec4<R>{ul[0] + dl[0], ul[1] + dl[1], ul[2] + dl[2], ul[3] + dl[3]};
         Rr = Vec4<R>{ur[0] + dr[0], ur[1] + dr[1], ur[2] + dr[2], ur[3] + dr[3]};
       }
-    } else if (e.r == -1) {
-      Rraw = far_ghost(WL, e.vx, e.vy, f);
+    } else if (ev.r == -1) {
+      Rraw = far_ghost(WL, vx, vy, f);
       Rr = Rraw;
     } else {
-      Rraw = wall_ghost(WL, e.vx, e.vy, e.vn);
+      Rraw = wall_ghost(WL, vx, vy, e.vn);
       Rr = Rraw;
     }
     Vec4<R> F = roe_flux(L, Rr, e.nx, e.ny, f.gamma);
@@ -195,7 +283,7 @@ __global__ void __launch_bounds__(256) flux_tile(TiledArgs<R> a) {
     F.dv *= e.vn;
     F.de *= e.vn;
     sF[s] = st(F);
-    sS[s] = spectral_radius(WL, Rraw, e.vx, e.vy, f.gamma);
+    sS[s] = spectral_radius(WL, Rraw, vx, vy, f.gamma);
   }
   __syncthreads();
 
@@ -203,7 +291,7 @@ __global__ void __launch_bounds__(256) flux_tile(TiledArgs<R> a) {
   R tmin = R(3.0e38);
   for (int i = threadIdx.x; i < nt; i += blockDim.x) {
     const int g = d.cell0 + i;
-    const int4 sl = a.slot[d.geo0 + g];
+    const int4 sl = (i == s0) ? sl0 : a.slot[d.geo0 + g];
     const int slk[3] = {sl.x, sl.y, sl.z};
     R T = 0;
     R r0 = R(0.0), r1 = R(0.0), r2 = R(0.0), r3 = R(0.0);
@@ -219,7 +307,7 @@ __global__ void __launch_bounds__(256) flux_tile(TiledArgs<R> a) {
       }
     }
     a.res[d.st0 + g] = SVec4<R>{r0, r1, r2, r3};
-    const R area = a.cg[d.geo0 + g].area;
+    const R area = (i == s0) ? area0 : a.cg[d.geo0 + g].area;
     if (a.global_dt) {
       const R v = area * f.cfl / T;
       a.dt[d.st0 + g] = v;
@@ -229,7 +317,7 @@ __global__ void __launch_bounds__(256) flux_tile(TiledArgs<R> a) {
     }
   }
   if (a.global_dt) {
-    __shared__ R red[256];
+    __shared__ R red[512];
     red[threadIdx.x] = tmin;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -254,12 +342,12 @@ __global__ void global_dt_min(const CaseCtl* ctl, const R* tile_min, R* dt_globa
 // Kernel B: smoothing + RK update for one tile.  LDS: R1[nl]
 // --------------------------------------------------------------------------
 template <typename R>
-__global__ void __launch_bounds__(256) update_tile(TiledArgs<R> a) {
+__global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int t = tile_of_block(blockIdx.x, a.ntiles);
   const TileDesc d = a.tiles[t];
   if (!a.ctl[d.icase].active) return;
-  const FlowT<R> f = load_flow(a.flow + d.icase);
+  const FlowT<R> f = a.flow[d.icase];
   const int nt = d.ncell, nl = d.ncell + d.nhalo;
   SVec4<R>* sR1 = reinterpret_cast<SVec4<R>*>(smem);
   const SVec4<R>* Rs = a.res + d.st0;
@@ -323,7 +411,7 @@ __global__ void __launch_bounds__(256) update_tile(TiledArgs<R> a) {
     }
   }
   if (a.last_stage) {
-    // fixed-order reduction: wave shuffles, then waves in order
+    // fixed-order tile sum: wave shuffles, then waves in order
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     __shared__ double wsum[16];
     const int wv = threadIdx.x >> 6;
@@ -336,11 +424,5 @@ __global__ void __launch_bounds__(256) update_tile(TiledArgs<R> a) {
     }
   }
 }
-
-// Per case: sum of the tile partials (fixed order, compensated), the
-// normalised residual Resd = sqrt((drou + 1e-32) / drou_step1) and the stop
-// test step < max_steps && Resd > epsl (gpu_solver.cu:2200, :2428-2433).
-__global__ void finalize_step(CaseCtl* ctl, const double* partial, double* hist, int hist_stride,
-                              int ncase);
 
 }  // namespace fvm

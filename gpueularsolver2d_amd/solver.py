@@ -18,7 +18,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
+# FVM_LIBRARY selects another build of the library (e.g. a variant under test)
+LIB_PATH = os.environ.get("FVM_LIBRARY", os.path.join(HERE, "libfvm_hip.so"))
 
 FVM_FP64, FVM_FP32 = 0, 1
 FVM_KERNELS_TILED, FVM_KERNELS_PLAIN = 0, 1

