@@ -1,0 +1,108 @@
+"""GPU batch engine and C-ABI behaviour: uneven batches equal single solves,
+per-case stopping, resumed runs, state round trip, failure reporting."""
+import numpy as np
+import pytest
+
+from gpueularsolver2d_amd import Batch, FvmError, Solver, naca0012_mesh
+from oracle_ctypes import OracleMesh, OracleSolver
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def meshes():
+    return [naca0012_mesh(3000, seed=11), naca0012_mesh(6000, seed=12)]
+
+
+FLOWS = [dict(mach=0.5, alpha_deg=0.0, cfl=1.5), dict(mach=0.8, alpha_deg=1.25, cfl=1.5),
+         dict(mach=0.7, alpha_deg=3.0, cfl=1.2)]
+
+
+@pytest.mark.parametrize("precision,kernels", [("fp64", "tiled"), ("fp32", "tiled"), ("fp64", "plain")])
+def test_batch_equals_single_solves(meshes, precision, kernels):
+    case_mesh = [0, 1, 1, 0, 1]
+    flows = [FLOWS[i % 3] for i in range(len(case_mesh))]
+    b = Batch(meshes, case_mesh, flows, precision=precision, kernels=kernels)
+    hist, steps, status = b.run(25)
+    assert list(steps) == [25] * 5 and not status.any()
+    cl, cd = b.forces()
+    for i, (m, f) in enumerate(zip(case_mesh, flows)):
+        s = Solver(meshes[m], precision=precision, kernels=kernels, **f)
+        h, n, st = s.run(25)
+        assert np.array_equal(b.state(i), s.state())
+        np.testing.assert_array_equal(hist[i, 1:26], h[1:])
+        assert (cl[i], cd[i]) == s.forces()
+
+
+def test_batch_fp64_matches_oracle(meshes):
+    b = Batch(meshes, [1, 0], [FLOWS[1], FLOWS[2]], precision="fp64")
+    b.run(15)
+    for i, (m, f) in enumerate([(1, FLOWS[1]), (0, FLOWS[2])]):
+        om = OracleMesh(meshes[m].x, meshes[m].y, meshes[m].tri)
+        o = OracleSolver(om, f["mach"], f["alpha_deg"], cfl=f["cfl"])
+        o.run(15)
+        assert np.array_equal(b.state(i), o.state())
+
+
+def test_per_case_stop(meshes):
+    # each case stops at its own first step with Resd <= epsl, as a single run would
+    b = Batch(meshes, [0, 1, 0], [FLOWS[0], FLOWS[1], FLOWS[2]], precision="fp64",
+              steps_per_graph=5)
+    hist, steps, status = b.run(400, epsl=0.2)
+    for i, (m, f) in enumerate([(0, FLOWS[0]), (1, FLOWS[1]), (0, FLOWS[2])]):
+        s = Solver(meshes[m], precision="fp64", **f)
+        h, n, st = s.run(400, epsl=0.2)
+        assert steps[i] == n
+        assert np.array_equal(b.state(i), s.state())
+    assert len(set(steps.tolist())) > 1
+
+
+def test_resume_equals_one_run(meshes):
+    a = Solver(meshes[0], precision="fp32", **FLOWS[1])
+    a.run(30)
+    h1, n1, _ = a.run(70)
+    b = Solver(meshes[0], precision="fp32", **FLOWS[1])
+    h2, n2, _ = b.run(70)
+    assert n1 == n2 == 70
+    assert np.array_equal(a.state(), b.state())
+    np.testing.assert_array_equal(h1[31:], h2[31:])
+
+
+def test_state_round_trip_and_restart(meshes):
+    s = Solver(meshes[1], precision="fp64", **FLOWS[2])
+    s.run(10)
+    W = s.state()
+    t = Solver(meshes[1], precision="fp64", **FLOWS[2])
+    t.set_state(W)
+    assert np.array_equal(t.state(), W)
+    # the FP32 path stores single precision
+    u = Solver(meshes[1], precision="fp32", **FLOWS[2])
+    u.set_state(W)
+    np.testing.assert_array_equal(u.state(), W.astype(np.float32).astype(np.float64))
+
+
+def test_nan_reported_not_silent(meshes):
+    # a non-physical state (negative density) forced from the host makes the
+    # residual non-finite: the run stops and reports FVM_ERR_NAN instead of
+    # ending silently as the reference loop does
+    s = Solver(meshes[0], precision="fp64", **FLOWS[0])
+    W = s.state()
+    W[100, 0] = -1.0
+    s.set_state(W)
+    with pytest.raises(FvmError) as e:
+        s.run(50)
+    assert e.value.name == "FVM_ERR_NAN"
+    h, n, st = s.run(50, check=False)
+    assert st == 5 and n < 50
+
+
+def test_kernel_times_and_config(meshes):
+    s = Solver(meshes[1], precision="fp32", tile_cells=64, threads=128, **FLOWS[1])
+    s.run(3)
+    ms = s.kernel_times(reps=5)
+    assert all(0 < v < 10 for v in ms)
+    cfg = s.config()
+    assert cfg["tile_cells"] == 64 and cfg["threads"] == 128 and cfg["tiles"] >= meshes[1].ncell // 64
+    W = s.state()
+    s.kernel_times(reps=3)  # timing launches leave the state unchanged
+    assert np.array_equal(s.state(), W)
