@@ -12,8 +12,9 @@ Measured in one run:
                    device synchronisation
   fp64_tiled / fp64_plain   the same workload in FP64 (tiled kernels; the
                    reference's per-phase kernel structure)
-  roofline         dominant kernel (flux_tile): algorithmic bytes per launch /
-                   average launch time from HIP events on the solver stream
+  roofline         dominant kernel (stage_tile, one launch per RK stage):
+                   algorithmic bytes per launch / average launch time from HIP
+                   events on the solver stream
   accuracy         FP32 vs FP64 (the FP64 path equals the FP64 oracle bit for
                    bit) after --acc-steps steps: |dCl|, |dCd|, density rel. L2
   cpu_baseline     the FP64 CPU oracle on this host, one thread, on a bounded
@@ -66,13 +67,16 @@ def timed_run(solver, warmup, steps):
     return t1 - t0, hist
 
 
-def traffic_from_profiles(workload, precision):
+def traffic_from_profiles(workload, precision, kernel):
+    """HBM bytes per launch measured with rocprofv3 counters (FETCH_SIZE +
+    WRITE_SIZE, each scaled by the same counter's reading for a copy kernel
+    of known size), recorded in profiles/traffic.json."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d.get(f"{workload}:{precision}:flux_tile")
+    return d.get(f"{workload}:{precision}:{kernel}")
 
 
 def cpu_baseline(mesh, steps):
@@ -102,8 +106,18 @@ def main():
     ap.add_argument("--fp32-only", action="store_true",
                     help="time only the FP32 path (for profiler runs)")
     ap.add_argument("--copy-mb", type=int, default=0,
-                    help="also run one device copy of this many MiB (calibrates HBM counters)")
+                    help="only run one device copy kernel of this many MiB and exit (calibrates "
+                         "the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_mb > 0:
+        import torch
+        x = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        torch.mul(x, 1.0, out=y)  # one kernel: reads and writes copy_mb MiB each
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_mib": args.copy_mb}))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU only: the solver has no multi-GPU path "
               "(independent cases would be split across GPUs by case; see DESIGN.md)",
@@ -121,15 +135,9 @@ def main():
     t32, _ = timed_run(s32, args.warmup, args.steps)
     kt = s32.kernel_times(reps=50)
     v32 = csu / t32 / 1e6
-    if args.copy_mb > 0:
-        import torch
-        x = torch.empty(args.copy_mb << 18, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        torch.mul(x, 1.0, out=y)  # one kernel: reads and writes copy_mb MiB each
-        torch.cuda.synchronize()
     if args.fp32_only:
         print(json.dumps({"metric": METRIC, "value": round(v32, 1), "unit": "Mcsu/s",
-                          "steps": args.steps, "flux_tile_ms": kt[0], "update_tile_ms": kt[1]}))
+                          "steps": args.steps, "kernel_ms": kt}))
         return 0
     s64 = Solver(mesh, precision="fp64", kernels="tiled", **flow)
     t64, _ = timed_run(s64, args.warmup, args.steps)
@@ -168,13 +176,16 @@ def main():
                    cl_fp64_range=float(np.ptp(smp[:, 2])))
 
     w = 4  # FP32 state and stored geometry
-    achieved = flux_bytes(w) * C / (kt[0] * 1e-3) / 1e9
-    roof = dict(bound="hbm", kernel="flux_tile", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+    if kt[1] > 0:   # two kernels per stage: flux_tile dominates
+        name, bytes_unit, t_kern = "flux_tile", flux_bytes(w), kt[0]
+    else:           # one kernel per stage carries all four sweeps
+        name, bytes_unit, t_kern = "stage_tile", flux_bytes(w) + update_bytes(w), kt[0]
+    achieved = bytes_unit * C / (t_kern * 1e-3) / 1e9
+    roof = dict(bound="hbm", kernel=name, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
                 unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
-                traffic=traffic_from_profiles(f"naca0012-{C}", "fp32"),
-                flux_tile_ms=kt[0], update_tile_ms=kt[1],
-                update_tile_gbs=round(update_bytes(w) * C / (kt[1] * 1e-3) / 1e9, 1),
-                bytes_per_csu_model=flux_bytes(w) + update_bytes(w),
+                traffic=traffic_from_profiles(f"naca0012-{C}", "fp32", name),
+                kernel_ms=t_kern, bytes_per_csu_model=flux_bytes(w) + update_bytes(w),
+                bytes_per_csu_kernel=bytes_unit,
                 step_gbs_model=round((flux_bytes(w) + update_bytes(w)) * v32 * 1e6 / 1e9, 1))
 
     cpu = None
@@ -192,7 +203,7 @@ def main():
                    "kernels": "tiled", "launch": s32.config()},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "fp64_tiled": {"value": round(v64, 1), "flux_tile_ms": kt64[0], "update_tile_ms": kt64[1]},
+        "fp64_tiled": {"value": round(v64, 1), "kernel_ms": kt64[0]},
         "fp64_plain": {"value": round(vpl, 1), "steps": plain_steps},
         "fp32_over_fp64_plain": round(v32 / vpl, 2),
         "accuracy": acc,

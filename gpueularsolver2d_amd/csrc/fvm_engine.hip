@@ -44,10 +44,10 @@ static const double kRK[4] = {0.0833, 0.2069, 0.4265, 1.0};  // gpu_fvm.cu:10
 // the tiles' partial sums before the close without a device-scope fence,
 // which on MI355X would write back a whole L2 per workgroup.
 __global__ void __launch_bounds__(64) finalize_kernel(CaseCtl* ctl, const double* partial,
-                                                      double* hist, int hist_stride) {
+                                                      double* hist, int hist_stride, int flip_w0) {
   CaseCtl& c = ctl[blockIdx.x];
   if (!c.active) return;
-  close_step(c, partial, hist, hist_stride, blockIdx.x);
+  close_step(c, partial, hist, hist_stride, blockIdx.x, flip_w0);
 }
 
 // ---------------------------------------------------------------------------
@@ -82,11 +82,12 @@ __device__ void forces_body(Load load, const WallTable w, int w0, int nw, const 
 }
 
 template <typename R>
-__global__ void forces_tiled(const SVec4<R>* W, const int* st0, const int* geo_w0, const int* nwall,
-                             WallTable w, const FlowT<double>* flow, double* cl, double* cd) {
+__global__ void forces_tiled(const SVec4<R>* Wa, const SVec4<R>* Wb, const CaseCtl* ctl,
+                             const int* st0, const int* geo_w0, const int* nwall, WallTable w,
+                             const FlowT<double>* flow, double* cl, double* cd) {
   const int c = blockIdx.x;
   if (threadIdx.x != 0) return;
-  const SVec4<R>* Wc = W + st0[c];
+  const SVec4<R>* Wc = (ctl[c].w0_cur ? Wb : Wa) + st0[c];
   auto load = [&](int i) {
     const SVec4<R> s = Wc[i];
     return Vec4<double>{(double)s.d, (double)s.du, (double)s.dv, (double)s.de};
@@ -178,6 +179,10 @@ class Batch {
   template <typename R>
   void enqueue_tiled_steps(int nsteps);
   template <typename R>
+  void enqueue_fused_steps(int nsteps);
+  template <typename R>
+  void fused_init();
+  template <typename R>
   void time_tiled(int reps, double ms[2]);
   void enqueue_plain_steps(int nsteps);
   void enqueue_steps(int nsteps);
@@ -193,7 +198,13 @@ class Batch {
   std::vector<CaseInfo> cases_;
   Arena arena_;
   int tile_cells_ = 0, threads_ = 256, steps_per_graph_ = 20, total_tiles_ = 0;
-  size_t lds_a_ = 0, lds_b_ = 0;
+  size_t lds_a_ = 0, lds_b_ = 0, lds_s_ = 0;
+  bool fused_ = false;      // one kernel per RK stage
+  bool r0_valid_ = false;   // (fused) the residual of stage 0 matches the current state
+  int max_ring_local_ = 0, max_w_local_ = 0;
+  void *d_res_b_ = nullptr, *d_dt_b_ = nullptr;
+  int* d_rings_ = nullptr;
+  int4* d_lnb_ = nullptr;
   // control
   CaseCtl* d_ctl_ = nullptr;
   std::vector<CaseCtl> h_ctl_;
@@ -256,8 +267,8 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
   if (o.order != 1 && o.order != 2) throw Error{FVM_ERR_ARGUMENT, "opts.order must be 1 or 2"};
   if (o.precision != FVM_FP64 && o.precision != FVM_FP32)
     throw Error{FVM_ERR_ARGUMENT, "opts.precision must be FVM_FP64 or FVM_FP32"};
-  if (o.kernels != FVM_KERNELS_TILED && o.kernels != FVM_KERNELS_PLAIN)
-    throw Error{FVM_ERR_ARGUMENT, "opts.kernels must be FVM_KERNELS_TILED or FVM_KERNELS_PLAIN"};
+  if (o.kernels != FVM_KERNELS_TILED && o.kernels != FVM_KERNELS_PLAIN && o.kernels != FVM_KERNELS_SPLIT)
+    throw Error{FVM_ERR_ARGUMENT, "opts.kernels must be FVM_KERNELS_TILED, _PLAIN or _SPLIT"};
   if (o.kernels == FVM_KERNELS_PLAIN && o.precision != FVM_FP64)
     throw Error{FVM_ERR_ARGUMENT, "the plain kernels run in FP64 only"};
   if (o.threads != 0 && (o.threads < 64 || o.threads > 512 || o.threads % 64 != 0))
@@ -278,7 +289,8 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
     ci.f64 = make_flow(flows[i], o.eps_smooth, &ci.rhoE_inf);
     cases_.push_back(ci);
   }
-  if (o.kernels == FVM_KERNELS_TILED) {
+  fused_ = o.kernels == FVM_KERNELS_TILED && !o.global_dt;
+  if (o.kernels != FVM_KERNELS_PLAIN) {
     // defaults from the launch-shape sweep on MI355X (autotune.py): 128-cell
     // tiles with 256 threads give one pass per phase for both precisions
     tile_cells_ = o.tile_cells > 0 ? o.tile_cells : 128;
@@ -324,24 +336,34 @@ void Batch::setup_tiled() {
     ML = std::max(ML, t.max_local);
     ME = std::max(ME, t.max_edges);
   }
+  for (auto& t : tiled_) {
+    max_ring_local_ = std::max(max_ring_local_, t.max_ring_local);
+    max_w_local_ = std::max(max_w_local_, t.max_w_local);
+  }
   lds_a_ = flux_tile_lds<R>(ML, ME);
   lds_b_ = update_tile_lds<R>(ML);
-  if (lds_a_ > 160 * 1024)
-    throw Error{FVM_ERR_ARGUMENT, "tile_cells too large: flux kernel needs " +
-                                      std::to_string(lds_a_) + " bytes of LDS"};
-  HIP_OK(set_tile_lds_limit<R>(std::max(lds_a_, lds_b_)));
+  lds_s_ = stage_tile_lds<R>(max_ring_local_, max_w_local_, ML, ME);
+  const size_t lds_max = std::max(lds_a_, fused_ ? lds_s_ : lds_b_);
+  if (lds_max > 160 * 1024)
+    throw Error{FVM_ERR_ARGUMENT, "tile_cells too large: the tile kernels need " +
+                                      std::to_string(lds_max) + " bytes of LDS"};
+  HIP_OK(set_tile_lds_limit<R>(std::max(lds_max, lds_b_)));
   // geometry offsets per mesh
-  std::vector<int> geo0(nmesh), halo0(nmesh), edge0(nmesh), wall0(nmesh);
-  int ncell_all = 0, nhalo_all = 0, nedge_all = 0, nwall_all = 0;
+  std::vector<int> geo0(nmesh), halo0(nmesh), edge0(nmesh), wall0(nmesh), ring0(nmesh), lnb0(nmesh);
+  int ncell_all = 0, nhalo_all = 0, nedge_all = 0, nwall_all = 0, nring_all = 0, nlnb_all = 0;
   for (int m = 0; m < nmesh; m++) {
     geo0[m] = ncell_all;
     halo0[m] = nhalo_all;
     edge0[m] = nedge_all;
     wall0[m] = nwall_all;
+    ring0[m] = nring_all;
+    lnb0[m] = nlnb_all;
     ncell_all += tiled_[m].ncell;
     nhalo_all += (int)tiled_[m].halo.size();
     nedge_all += (int)tiled_[m].e_l.size();
     nwall_all += (int)tiled_[m].w_cell.size();
+    nring_all += (int)tiled_[m].rings.size();
+    nlnb_all += (int)tiled_[m].lnb.size() / 3;
   }
   // case state offsets and the batch tile list
   std::vector<TileDesc> tiles;
@@ -367,6 +389,10 @@ void Batch::setup_tiled() {
       d.nhalo = T.tile_halo0[k + 1] - T.tile_halo0[k];
       d.edge0 = edge0[m] + T.tile_edge0[k];
       d.nedge = T.tile_edge0[k + 1] - T.tile_edge0[k];
+      d.ring0 = ring0[m] + T.tile_ring0[k];
+      d.n2 = T.ring_n2[k];
+      d.n3 = T.ring_n3[k];
+      d.lnb0 = lnb0[m] + T.tile_lnb0[k];
       tiles.push_back(d);
     }
     ncell_state += T.ncell;
@@ -397,6 +423,12 @@ void Batch::setup_tiled() {
     d_wc_ = arena_.take<SVec4<R>>(ncell_state);
     d_res_ = arena_.take<SVec4<R>>(ncell_state);
     d_dt_ = arena_.take<R>(ncell_state);
+    if (fused_) {
+      d_res_b_ = arena_.take<SVec4<R>>(ncell_state);
+      d_dt_b_ = arena_.take<R>(ncell_state);
+      d_rings_ = arena_.take<int>(nring_all);
+      d_lnb_ = arena_.take<int4>(nlnb_all);
+    }
     d_partial_ = arena_.take<double>(tiles.size());
     d_tile_min_ = arena_.take<R>(tiles.size());
     d_dtg_ = arena_.take<R>(ncase);
@@ -428,8 +460,15 @@ void Batch::setup_tiled() {
   std::vector<EdgeRec<R>> hedge(nedge_all);
   std::vector<int> hwcell(nwall_all);
   std::vector<double> hwvx(nwall_all), hwvy(nwall_all), hwvn(nwall_all);
+  std::vector<int> hrings(fused_ ? nring_all : 0);
+  std::vector<int4> hlnb(fused_ ? nlnb_all : 0);
   for (int m = 0; m < nmesh; m++) {
     const TiledMesh& T = tiled_[m];
+    if (fused_) {
+      std::copy(T.rings.begin(), T.rings.end(), hrings.begin() + ring0[m]);
+      for (size_t q = 0; q < T.lnb.size() / 3; q++)
+        hlnb[lnb0[m] + q] = make_int4(T.lnb[3 * q], T.lnb[3 * q + 1], T.lnb[3 * q + 2], 0);
+    }
     for (int i = 0; i < T.ncell; i++) {
       CellGeom<R>& g = hcg[geo0[m] + i];
       for (int k = 0; k < 3; k++) {
@@ -465,6 +504,10 @@ void Batch::setup_tiled() {
     hst0[c] = state_off_[c];
     hw0off[c] = wall0[m];
     hnw[c] = (int)tiled_[m].w_cell.size();
+  }
+  if (fused_) {
+    h2d(d_rings_, hrings, stream_);
+    h2d(d_lnb_, hlnb, stream_);
   }
   h2d(flow, hflow, stream_);
   h2d(d_flow64_, hflow64, stream_);
@@ -672,7 +715,53 @@ TiledArgs<R> Batch::tiled_args() {
   }
   a.max_local = ML;
   a.max_edges = ME;
+  a.w0a = static_cast<SVec4<R>*>(d_w0_);
+  a.w0b = static_cast<SVec4<R>*>(d_wc_);
+  a.rings = d_rings_;
+  a.lnb = d_lnb_;
+  a.max_ring_local = max_ring_local_;
+  a.max_w_local = max_w_local_;
   return a;
+}
+
+// One kernel per stage.  Residual and time-step buffers alternate between
+// (a) and (b) across the stages; a step is: stages 1..3 (each applies the
+// previous stage's update), then the launch that applies the last update,
+// writes the new step-start state and computes stage 0 of the next step, then
+// the step close.  Stage 0 of the very first step comes from fused_init.
+template <typename R>
+void Batch::enqueue_fused_steps(int nsteps) {
+  TiledArgs<R> a = tiled_args<R>();
+  SVec4<R>* Ra = static_cast<SVec4<R>*>(d_res_);
+  SVec4<R>* Rb = static_cast<SVec4<R>*>(d_res_b_);
+  R* Da = static_cast<R*>(d_dt_);
+  R* Db = static_cast<R*>(d_dt_b_);
+  a.apply_update = 1;
+  for (int s = 0; s < nsteps; s++) {
+    for (int k = 1; k <= 4; k++) {
+      const bool from_a = (k & 1) == 1;  // stages 1 and 3 read (a)
+      a.r_in = from_a ? Ra : Rb;
+      a.dt_in = from_a ? Da : Db;
+      a.res = from_a ? Rb : Ra;
+      a.dt = from_a ? Db : Da;
+      a.rk = (R)kRK[k - 1];
+      a.close_step = k == 4;
+      HIP_OK(launch_stage_tile<R>(a, threads_, lds_s_, stream_));
+    }
+    hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(64), 0, stream_, d_ctl_,
+                       d_partial_, d_hist_, hist_stride_, 1);
+    HIP_OK(hipGetLastError());
+  }
+}
+
+template <typename R>
+void Batch::fused_init() {
+  TiledArgs<R> a = tiled_args<R>();
+  a.apply_update = 0;
+  a.close_step = 0;
+  a.res = static_cast<SVec4<R>*>(d_res_);
+  a.dt = static_cast<R*>(d_dt_);
+  HIP_OK(launch_stage_tile<R>(a, threads_, lds_s_, stream_));
 }
 
 template <typename R>
@@ -693,35 +782,46 @@ void Batch::enqueue_tiled_steps(int nsteps) {
       HIP_OK(launch_update_tile<R>(a, threads_, lds_b_, stream_));
     }
     hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(64), 0, stream_, d_ctl_,
-                       d_partial_, d_hist_, hist_stride_);
+                       d_partial_, d_hist_, hist_stride_, 0);
     HIP_OK(hipGetLastError());
   }
 }
 
 template <typename R>
 void Batch::time_tiled(int reps, double ms[2]) {
-  // Second-stage arguments: reads Wc and W0, writes R, dt and Wc only; the
-  // update is W0 - a dt R / A of unchanged inputs, so repeating it is
-  // idempotent and the next stage overwrites Wc anyway.
+  // Second-stage arguments.  Split path: reads Wc and W0, writes R, dt and
+  // Wc.  One kernel per stage: reads the (a) residual and time step and the
+  // step-start state, writes the (b) residual and time step.  Each launch
+  // recomputes the same outputs from unchanged inputs, and the next run
+  // overwrites them before use, so the solution is unaffected.
   TiledArgs<R> a = tiled_args<R>();
   a.w_in = static_cast<SVec4<R>*>(d_wc_);
   a.w_out = static_cast<SVec4<R>*>(d_wc_);
   a.rk = (R)kRK[1];
   a.last_stage = 0;
+  a.apply_update = 1;
+  a.close_step = 0;
+  a.r_in = static_cast<SVec4<R>*>(d_res_);
+  a.dt_in = static_cast<R*>(d_dt_);
+  if (fused_) {
+    a.res = static_cast<SVec4<R>*>(d_res_b_);
+    a.dt = static_cast<R*>(d_dt_b_);
+  }
   // every case active for the measurement, then restore the control block
   std::vector<CaseCtl> on = h_ctl_;
   for (auto& c : on) c.active = 1;
   HIP_OK(hipMemcpyAsync(d_ctl_, on.data(), on.size() * sizeof(CaseCtl), hipMemcpyHostToDevice, stream_));
-  HIP_OK(launch_flux_tile<R>(a, threads_, lds_a_, stream_));
-  HIP_OK(launch_update_tile<R>(a, threads_, lds_b_, stream_));
-  for (int k = 0; k < 2; k++) {
+  auto launch = [&](int k) {
+    if (fused_) return launch_stage_tile<R>(a, threads_, lds_s_, stream_);
+    return k == 0 ? launch_flux_tile<R>(a, threads_, lds_a_, stream_)
+                  : launch_update_tile<R>(a, threads_, lds_b_, stream_);
+  };
+  const int nk = fused_ ? 1 : 2;
+  ms[1] = 0.0;
+  for (int k = 0; k < nk; k++) {
+    HIP_OK(launch(k));
     HIP_OK(hipEventRecord(ev0_, stream_));
-    for (int r = 0; r < reps; r++) {
-      if (k == 0)
-        HIP_OK(launch_flux_tile<R>(a, threads_, lds_a_, stream_));
-      else
-        HIP_OK(launch_update_tile<R>(a, threads_, lds_b_, stream_));
-    }
+    for (int r = 0; r < reps; r++) HIP_OK(launch(k));
     HIP_OK(hipEventRecord(ev1_, stream_));
     HIP_OK(hipEventSynchronize(ev1_));
     float t = 0.f;
@@ -734,7 +834,7 @@ void Batch::time_tiled(int reps, double ms[2]) {
 
 void Batch::kernel_times(int reps, double ms[2]) {
   HIP_OK(hipSetDevice(device_));
-  if (o_.kernels != FVM_KERNELS_TILED) throw Error{FVM_ERR_ARGUMENT, "kernel timing covers the tiled kernels only"};
+  if (o_.kernels == FVM_KERNELS_PLAIN) throw Error{FVM_ERR_ARGUMENT, "kernel timing covers the tiled kernels only"};
   if (reps < 1) throw Error{FVM_ERR_ARGUMENT, "reps must be >= 1"};
   if (o_.precision == FVM_FP64)
     time_tiled<double>(reps, ms);
@@ -749,7 +849,7 @@ void Batch::enqueue_plain_steps(int nsteps) {
                         stream_);
     HIP_OK(hipGetLastError());
     hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(64), 0, stream_, d_ctl_,
-                       d_partial_, d_hist_, hist_stride_);
+                       d_partial_, d_hist_, hist_stride_, 0);
     HIP_OK(hipGetLastError());
   }
 }
@@ -757,6 +857,10 @@ void Batch::enqueue_plain_steps(int nsteps) {
 void Batch::enqueue_steps(int nsteps) {
   if (o_.kernels == FVM_KERNELS_PLAIN)
     enqueue_plain_steps(nsteps);
+  else if (fused_ && o_.precision == FVM_FP64)
+    enqueue_fused_steps<double>(nsteps);
+  else if (fused_)
+    enqueue_fused_steps<float>(nsteps);
   else if (o_.precision == FVM_FP64)
     enqueue_tiled_steps<double>(nsteps);
   else
@@ -806,6 +910,13 @@ void Batch::run(int max_steps, double epsl, double* hist, int* steps_done, int* 
   }
   if (!graph_) build_graph();
   HIP_OK(hipMemcpyAsync(d_ctl_, h_ctl_.data(), ncase * sizeof(CaseCtl), hipMemcpyHostToDevice, stream_));
+  if (fused_ && !r0_valid_) {
+    if (o_.precision == FVM_FP64)
+      fused_init<double>();
+    else
+      fused_init<float>();
+    r0_valid_ = true;
+  }
   HIP_OK(hipEventRecord(ev0_, stream_));
   for (;;) {
     int min_step = max_steps, any = 0;
@@ -859,10 +970,11 @@ void Batch::get_state(int icase, double* rho, double* rhou, double* rhov, double
     return;
   }
   const TiledMesh& T = tiled_[cases_[icase].mesh];
+  auto state_buffer = [&]() { return h_ctl_[icase].w0_cur ? d_wc_ : d_w0_; };
   auto fetch = [&](auto tag) {
     using R = decltype(tag);
     std::vector<SVec4<R>> h(C);
-    HIP_OK(hipMemcpy(h.data(), static_cast<SVec4<R>*>(d_w0_) + state_off_[icase],
+    HIP_OK(hipMemcpy(h.data(), static_cast<SVec4<R>*>(state_buffer()) + state_off_[icase],
                      C * sizeof(SVec4<R>), hipMemcpyDeviceToHost));
     for (int n = 0; n < C; n++) {
       const int o = T.new_to_old[n];
@@ -891,6 +1003,7 @@ void Batch::set_state(int icase, const double* rho, const double* rhou, const do
     return;
   }
   const TiledMesh& T = tiled_[cases_[icase].mesh];
+  auto state_buffer = [&]() { return h_ctl_[icase].w0_cur ? d_wc_ : d_w0_; };
   auto put = [&](auto tag) {
     using R = decltype(tag);
     std::vector<SVec4<R>> h(C);
@@ -898,9 +1011,10 @@ void Batch::set_state(int icase, const double* rho, const double* rhou, const do
       const int o = T.new_to_old[n];
       h[n] = SVec4<R>{(R)rho[o], (R)rhou[o], (R)rhov[o], (R)rhoe[o]};
     }
-    HIP_OK(hipMemcpy(static_cast<SVec4<R>*>(d_w0_) + state_off_[icase], h.data(),
+    HIP_OK(hipMemcpy(static_cast<SVec4<R>*>(state_buffer()) + state_off_[icase], h.data(),
                      C * sizeof(SVec4<R>), hipMemcpyHostToDevice));
   };
+  r0_valid_ = false;
   if (o_.precision == FVM_FP64)
     put(double{});
   else
@@ -916,12 +1030,12 @@ void Batch::forces(double* cl, double* cd) {
                          topo_[cases_[c].mesh].nwall, d_flow64_ + c, d_cl_ + c, d_cd_ + c);
   } else if (o_.precision == FVM_FP64) {
     hipLaunchKernelGGL(forces_tiled<double>, dim3(ncase), dim3(64), 0, stream_,
-                       static_cast<const SVec4<double>*>(d_w0_), d_case_st0_, d_case_w0_, d_case_nw_,
-                       walls_, d_flow64_, d_cl_, d_cd_);
+                       static_cast<const SVec4<double>*>(d_w0_), static_cast<const SVec4<double>*>(d_wc_),
+                       d_ctl_, d_case_st0_, d_case_w0_, d_case_nw_, walls_, d_flow64_, d_cl_, d_cd_);
   } else {
     hipLaunchKernelGGL(forces_tiled<float>, dim3(ncase), dim3(64), 0, stream_,
-                       static_cast<const SVec4<float>*>(d_w0_), d_case_st0_, d_case_w0_, d_case_nw_,
-                       walls_, d_flow64_, d_cl_, d_cd_);
+                       static_cast<const SVec4<float>*>(d_w0_), static_cast<const SVec4<float>*>(d_wc_),
+                       d_ctl_, d_case_st0_, d_case_w0_, d_case_nw_, walls_, d_flow64_, d_cl_, d_cd_);
   }
   HIP_OK(hipGetLastError());
   if (cl) HIP_OK(hipMemcpyAsync(cl, d_cl_, ncase * sizeof(double), hipMemcpyDeviceToHost, stream_));

@@ -13,6 +13,8 @@ hipError_t launch_flux_tile(const TiledArgs<R>& a, int threads, size_t lds, hipS
 template <typename R>
 hipError_t launch_update_tile(const TiledArgs<R>& a, int threads, size_t lds, hipStream_t s);
 template <typename R>
+hipError_t launch_stage_tile(const TiledArgs<R>& a, int threads, size_t lds, hipStream_t s);
+template <typename R>
 hipError_t launch_global_dt_min(const CaseCtl* ctl, const R* tile_min, R* dt_global, int ncase,
                                 hipStream_t s);
 // Largest dynamic LDS the two tile kernels may use (bytes).
@@ -27,6 +29,11 @@ inline size_t flux_tile_lds(int max_local, int max_edges) {
 template <typename R>
 inline size_t update_tile_lds(int max_local) {
   return (size_t)max_local * sizeof(SVec4<R>);
+}
+template <typename R>
+inline size_t stage_tile_lds(int max_ring_local, int max_w_local, int max_local, int max_edges) {
+  return (size_t)(max_ring_local + 2 * max_w_local) * sizeof(SVec4<R>) +
+         flux_tile_lds<R>(max_local, max_edges) - (size_t)max_local * sizeof(SVec4<R>);
 }
 
 }  // namespace fvm

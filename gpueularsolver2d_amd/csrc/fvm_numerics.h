@@ -121,6 +121,16 @@ __device__ __forceinline__ T limit(T delt, T u, T umax, T umin) {
   return delt * fi;
 }
 
+// FP32: the same limited slope without a division.  For delta > 0,
+// delta * min((Umax-U)/delta, 1) = min(Umax-U, delta); for delta < 0 the
+// product with a negative delta turns the min into max(Umin-U, delta).
+template <>
+__device__ __forceinline__ float limit<float>(float delt, float u, float umax, float umin) {
+  if (delt > 0.f) return fminf(umax - u, delt);
+  if (delt < 0.f) return fmaxf(umin - u, delt);
+  return delt;
+}
+
 // kernel_Calc_Function, gpu_solver.cu:552-566 (reciprocal-multiply form).
 template <typename T>
 __device__ __forceinline__ void calc_function(const Vec4<T>& w, T gamma, T& U, T& V, T& H, T& P) {
@@ -194,6 +204,53 @@ __device__ __forceinline__ Vec4<T> roe_flux(const Vec4<T>& LW, const Vec4<T>& RW
   return F;
 }
 
+// FP32 form of the same Roe flux: identical formulas, with shared
+// reciprocals (one of sqrt(rho_L)+sqrt(rho_R), one of the squared Roe sound
+// speed) and the entropy-fix division by 2*delta folded into a constant.
+template <>
+__device__ __forceinline__ Vec4<float> roe_flux<float>(const Vec4<float>& LW, const Vec4<float>& RW,
+                                                       float nx, float ny, float gamma) {
+  const float gm1 = gamma - 1.0f;
+  const float rl = 1.0f / LW.d, rr = 1.0f / RW.d;
+  const float U0 = LW.du * rl, V0 = LW.dv * rl, U1 = RW.du * rr, V1 = RW.dv * rr;
+  const float P0 = gm1 * (LW.de - 0.5f * LW.d * (U0 * U0 + V0 * V0));
+  const float P1 = gm1 * (RW.de - 0.5f * RW.d * (U1 * U1 + V1 * V1));
+  const float H0 = (LW.de + P0) * rl, H1 = (RW.de + P1) * rr;
+  const float M0 = U0 * nx + V0 * ny, M1 = U1 * nx + V1 * ny, DVC = M1 - M0;
+  const float sl = sqrtf(LW.d), sr = sqrtf(RW.d);
+  const float ROUB = sl * sr;
+  const float is = 1.0f / (sl + sr);
+  const float wl = sl * is, wr = sr * is;
+  const float UB = U0 * wl + U1 * wr, VB = V0 * wl + V1 * wr, HB = H0 * wl + H1 * wr;
+  const float QB = UB * UB + VB * VB;
+  const float AB2 = gm1 * (HB - 0.5f * QB);
+  const float AB = sqrtf(AB2);
+  const float iAB2 = 1.0f / AB2;
+  const float VC = UB * nx + VB * ny;
+  const float e = 0.05f, e2 = 0.05f * 0.05f, i2e = 1.0f / (2.0f * 0.05f);
+  const float l0 = VC - AB, l2 = VC + AB;
+  const float AD0 = fabsf(l0) < e ? (l0 * l0 + e2) * i2e : fabsf(l0);
+  const float AD1 = fabsf(VC) < e ? (VC * VC + e2) * i2e : fabsf(VC);
+  const float AD2 = fabsf(l2) < e ? (l2 * l2 + e2) * i2e : fabsf(l2);
+  const float dP = P1 - P0, rad = ROUB * AB * DVC, h = 0.5f * iAB2;
+  const float F00 = AD0 * (dP - rad) * h;
+  const float F20 = AD2 * (dP + rad) * h;
+  const float F10 = AD1 * ((RW.d - LW.d) - dP * iAB2);
+  const float ar = AD1 * ROUB;
+  const float dU = U1 - U0, dV = V1 - V0;
+  const float WL0 = LW.d * M0, WR0 = RW.d * M1;
+  Vec4<float> F;
+  F.d = 0.5f * ((WL0 + WR0) - (F00 + F10 + F20));
+  F.du = 0.5f * ((U0 * WL0 + nx * P0 + U1 * WR0 + nx * P1) -
+                 (F00 * (UB - AB * nx) + F10 * UB + ar * (dU - DVC * nx) + F20 * (UB + AB * nx)));
+  F.dv = 0.5f * ((V0 * WL0 + ny * P0 + V1 * WR0 + ny * P1) -
+                 (F00 * (VB - AB * ny) + F10 * VB + ar * (dV - DVC * ny) + F20 * (VB + AB * ny)));
+  F.de = 0.5f * ((WL0 * H0 + WR0 * H1) -
+                 (F00 * (HB - AB * VC) + F10 * QB * 0.5f + ar * (UB * dU + VB * dV - DVC * VC) +
+                  F20 * (HB + AB * VC)));
+  return F;
+}
+
 // kernel_deltT body, gpu_solver.cu:434-442: spectral radius of an edge from the
 // arithmetic mean of the unreconstructed left and right states.
 template <typename T>
@@ -207,6 +264,19 @@ __device__ __forceinline__ T spectral_radius(const Vec4<T>& L, const Vec4<T>& R,
   T p = rou * EE * (gamma - T(1.0));
   T CS = sqrt((gamma * p / rou) * (vx * vx + vy * vy));
   return fabs(VC) + CS;
+}
+
+// FP32: the edge vector enters as (vx, vy) = (ny, nx) |v| (see EdgeRec<float>),
+// so |u vy + v vx| + sqrt(gamma p / rho) |v| is formed from the unit normal.
+__device__ __forceinline__ float spectral_radius_n(const Vec4<float>& L, const Vec4<float>& R,
+                                                   float nx, float ny, float vn, float gamma) {
+  const float rou = (L.d + R.d) * 0.5f;
+  const float ir = 1.0f / rou;
+  const float u = (L.du + R.du) * 0.5f * ir;
+  const float v = (L.dv + R.dv) * 0.5f * ir;
+  const float EE = (L.de + R.de) * 0.5f * ir - (u * u + v * v) * 0.5f;
+  const float a2 = gamma * (gamma - 1.0f) * EE;  // gamma p / rho with p = rho EE (gamma - 1)
+  return (fabsf(u * nx + v * ny) + sqrtf(a2)) * vn;
 }
 
 }  // namespace fvm

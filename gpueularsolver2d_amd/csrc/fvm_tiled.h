@@ -1,6 +1,22 @@
-// Tiled kernels of the explicit finite-volume Euler path: two launches per
-// Runge-Kutta stage, each covering every tile of every case of a batch.
+// Tiled kernels of the explicit finite-volume Euler path.  Every launch
+// covers every tile of every case of a batch, one workgroup per tile.
 //
+// One kernel per Runge-Kutta stage (stage_tile, the default): the RK update
+// of the previous stage is recomputed for the tile and two rings of
+// neighbours (so the updated state never leaves LDS), then the gradient,
+// flux and gather of this stage run as in kernel A below.  In order:
+//   1. smoothing sweep 1 of the previous stage's residual on the tile and
+//      rings 1..3 (kernel_Smooth1, gpu_solver.cu:1635-1673);
+//   2. sweep 2, selection and W = W0 - a dt R / A on the tile and rings 1..2
+//      (kernel_Smooth2 :1675-1720, kernel_RKStep :938-966); in the stage that
+//      closes a time step the tile's new W0 and its FP64 partial of
+//      (rho - rho0)^2 (kernel_drou :968-1020) are written;
+//   3.-5. gradient, flux and gather as in kernel A.
+// Cells of the rings are updated by several tiles from identical inputs, so
+// every tile sees bitwise the same values.
+//
+// Two kernels per stage (flux_tile + update_tile), used with the global time
+// step, which needs a grid-wide minimum between flux and update:
 // Kernel A (flux_tile): one workgroup per tile of cells.
 //   1. Green-Gauss gradient and neighbourhood bounds of the tile's cells and
 //      of its first-ring halo, into LDS (kernel_Gradient, gpu_solver.cu:
@@ -69,6 +85,17 @@ __host__ __device__ inline EdgeRec<float> make_edge_rec(double nx, double ny, do
                         (float)rry, (int)(((unsigned)r << 16) | ((unsigned)l & 0xffffu))};
 }
 
+// Spectral radius of a tile edge: FP64 keeps the reference's form on the
+// edge vector, FP32 uses the unit normal and length.
+__device__ __forceinline__ double edge_sr(const EdgeRec<double>&, const Vec4<double>& L,
+                                          const Vec4<double>& R, double vx, double vy, double g) {
+  return spectral_radius(L, R, vx, vy, g);
+}
+__device__ __forceinline__ float edge_sr(const EdgeRec<float>& e, const Vec4<float>& L,
+                                         const Vec4<float>& R, float, float, float g) {
+  return spectral_radius_n(L, R, e.nx, e.ny, e.vn, g);
+}
+
 struct EdgeView {
   int l, r;
 };
@@ -94,17 +121,27 @@ struct TiledArgs {
   const int* halo;
   const EdgeRec<R>* edges;
   const SVec4<R>* w_in;       // state read by kernel A
-  SVec4<R>* res;              // per case cell
-  R* dt;                      // per case cell (local dt, or A*CFL/T in global mode)
+  SVec4<R>* res;              // residual written by kernel A / stage_tile
+  R* dt;                      // local dt written by kernel A / stage_tile (A*CFL/T in global mode)
   R* tile_min;                // per tile minimum of A*CFL/T (global dt mode)
   const R* dt_global;         // per case (global dt mode)
-  const SVec4<R>* w0;         // step-start state
+  const SVec4<R>* w0;         // step-start state (two-kernel path)
   SVec4<R>* w_out;            // kernel B output
   double* partial;            // per tile sum of (rho - rho0)^2
-  R rk;                       // RK stage coefficient
-  int last_stage;             // 1 in the fourth stage
+  R rk;                       // RK coefficient of the update applied by this launch
+  int last_stage;             // kernel B: 1 in the fourth stage
   int order, smoothing, global_dt;
   int max_local, max_edges;
+  // one-kernel-per-stage path
+  SVec4<R>* w0a;              // the two step-start state buffers (CaseCtl::w0_cur selects)
+  SVec4<R>* w0b;
+  const SVec4<R>* r_in;       // residual of the previous stage
+  const R* dt_in;             // time step of the previous stage
+  const int* rings;           // rings 1..3 of every tile
+  const int4* lnb;            // tile-local neighbours of tile + rings 1..2
+  int apply_update;           // 0 only for the first launch of a run
+  int close_step;             // 1: the update closes a time step (new W0 + partial)
+  int max_ring_local, max_w_local;
 };
 
 __device__ __forceinline__ int tile_of_block(int bid, int ntiles) {
@@ -139,8 +176,9 @@ __device__ __forceinline__ void neumaier(double& s, double& c, double v) {
 // normalised residual Resd = sqrt((drou + 1e-32) / drou_of_step_1), the
 // history entry and the stop test step < max_steps && Resd > epsl
 // (gpu_solver.cu:2200, :2403-2433).  Called by one full 64-lane wave.
+// flip_w0: the step's new state went to the other step-start buffer.
 __device__ inline void close_step(CaseCtl& c, const double* partial, double* hist, int hist_stride,
-                                  int icase) {
+                                  int icase, int flip_w0) {
   const int lane = threadIdx.x & 63;
   const int n = c.ntile, per = (n + 63) / 64;
   const int b = lane * per, e = min(n, b + per);
@@ -154,6 +192,7 @@ __device__ inline void close_step(CaseCtl& c, const double* partial, double* his
   if (lane == 0) {
     const int step = c.step + 1;
     c.step = step;
+    if (flip_w0) c.w0_cur ^= 1;
     if (step == 1) c.epsm = drou;
     const double resd = sqrt((drou + 1.0E-32) / c.epsm);
     c.resd = resd;
@@ -163,86 +202,59 @@ __device__ inline void close_step(CaseCtl& c, const double* partial, double* his
   }
 }
 
-// --------------------------------------------------------------------------
-// Kernel A: gradient + flux + gather for one tile.
-// LDS: W[nl] | grad[8][nl] | bounds[8][nl] | F[ne] | SR[ne]
-// --------------------------------------------------------------------------
-template <typename R>
-__global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int t = tile_of_block(blockIdx.x, a.ntiles);
-  const TileDesc d = a.tiles[t];
-  if (!a.ctl[d.icase].active) return;
-  const FlowT<R> f = a.flow[d.icase];
-  const int nt = d.ncell, nl = d.ncell + d.nhalo, ne = d.nedge;
-  const int ML = a.max_local, ME = a.max_edges;
-  SVec4<R>* sW = reinterpret_cast<SVec4<R>*>(smem);
-  R* sG = reinterpret_cast<R*>(sW + ML);  // [8][ML]: d/dx 4 comps, d/dy 4 comps
-  R* sB = sG + 8 * ML;                    // [8][ML]: max 4 comps, min 4 comps
-  SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
-  R* sS = reinterpret_cast<R*>(sF + ME);
-  const SVec4<R>* W = a.w_in + d.st0;
+// ---------------------------------------------------------------------------
+// Phases shared by flux_tile and stage_tile
+// ---------------------------------------------------------------------------
 
-  // Prefetch the first edge record and gather slot of this thread; their
-  // latency then hides behind the gradient phase.
-  const int s0 = threadIdx.x;
-  EdgeRec<R> e0{};
-  if (s0 < ne) e0 = a.edges[d.edge0 + s0];
-  int4 sl0 = make_int4(0, 0, 0, 0);
-  R area0 = R(0);
-  if (s0 < nt) {
-    sl0 = a.slot[d.geo0 + d.cell0 + s0];
-    area0 = a.cg[d.geo0 + d.cell0 + s0].area;
-  }
-
-  // 1. gradient and bounds of tile + halo cells
-  for (int j = threadIdx.x; j < nl; j += blockDim.x) {
-    const int g = j < nt ? d.cell0 + j : a.halo[d.halo0 + j - nt];
-    const Vec4<R> Wi = ld(W[g]);
-    sW[j] = st(Wi);
-    if (a.order >= 2) {
-      const CellGeom<R> cg = a.cg[d.geo0 + g];
-      const int4 nb = a.nbr[d.geo0 + g];
-      const int nbk[3] = {nb.x, nb.y, nb.z};
-      R gx[4] = {R(0.0), R(0.0), R(0.0), R(0.0)}, gy[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
-      R mx[4], mn[4];
-      const R ui[4] = {Wi.d, Wi.du, Wi.dv, Wi.de};
+// Green-Gauss gradient and neighbourhood bounds of one cell into the LDS
+// arrays sG[8][ML], sB[8][ML] (kernel_Gradient, gpu_solver.cu:225-383).
+// nbk: neighbour codes (>= 0 interior, -1 far field, -2 wall); fetch(k)
+// returns the state of interior neighbour k.
+template <typename R, typename Fetch>
+__device__ __forceinline__ void gradient_cell(int j, const Vec4<R>& Wi, const CellGeom<R>& cg,
+                                              const int nbk[3], Fetch fetch, const FlowT<R>& f,
+                                              R* sG, R* sB, int ML) {
+  R gx[4] = {R(0.0), R(0.0), R(0.0), R(0.0)}, gy[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
+  R mx[4], mn[4];
+  const R ui[4] = {Wi.d, Wi.du, Wi.dv, Wi.de};
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        Vec4<R> c;
-        if (nbk[k] >= 0)
-          c = ld(W[nbk[k]]);
-        else
-          c = boundary_ghost<R>(-nbk[k], Wi, cg.vx[k], cg.vy[k], f);
-        const R cc[4] = {c.d, c.du, c.dv, c.de};
+  for (int k = 0; k < 3; k++) {
+    const Vec4<R> c = nbk[k] >= 0 ? fetch(k) : boundary_ghost<R>(-nbk[k], Wi, cg.vx[k], cg.vy[k], f);
+    const R cc[4] = {c.d, c.du, c.dv, c.de};
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          if (k == 0) {
-            mx[q] = cc[q];
-            mn[q] = cc[q];
-          } else {
-            if (cc[q] < mn[q]) mn[q] = cc[q];
-            if (cc[q] > mx[q]) mx[q] = cc[q];
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q++) gx[q] += R(0.5) * (ui[q] + cc[q]) * cg.vy[k];
-#pragma unroll
-        for (int q = 0; q < 4; q++) gy[q] += R(0.5) * (ui[q] + cc[q]) * cg.vx[k];
-      }
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        sG[q * ML + j] = gx[q] * cg.inv_area;
-        sG[(4 + q) * ML + j] = gy[q] * cg.inv_area;
-        sB[q * ML + j] = fmax(ui[q], mx[q]);
-        sB[(4 + q) * ML + j] = fmin(ui[q], mn[q]);
+    for (int q = 0; q < 4; q++) {
+      if (k == 0) {
+        mx[q] = cc[q];
+        mn[q] = cc[q];
+      } else {
+        if (cc[q] < mn[q]) mn[q] = cc[q];
+        if (cc[q] > mx[q]) mx[q] = cc[q];
       }
     }
+#pragma unroll
+    for (int q = 0; q < 4; q++) gx[q] += R(0.5) * (ui[q] + cc[q]) * cg.vy[k];
+#pragma unroll
+    for (int q = 0; q < 4; q++) gy[q] += R(0.5) * (ui[q] + cc[q]) * cg.vx[k];
   }
-  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    sG[q * ML + j] = gx[q] * cg.inv_area;
+    sG[(4 + q) * ML + j] = gy[q] * cg.inv_area;
+    sB[q * ML + j] = fmax(ui[q], mx[q]);
+    sB[(4 + q) * ML + j] = fmin(ui[q], mn[q]);
+  }
+}
 
-  // 2. flux and spectral radius of the tile's edges
-  for (int s = threadIdx.x; s < ne; s += blockDim.x) {
+// Roe flux and spectral radius of every edge of the tile into LDS slots
+// (kernel_Flux2 gpu_solver.cu:651-764 with kernel_LBJinterpolate :496-549,
+// kernel_deltT :385-454).  e0 is this thread's prefetched first record.
+template <typename R>
+__device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc& d,
+                                           const FlowT<R>& f, const SVec4<R>* sW, const R* sG,
+                                           const R* sB, SVec4<R>* sF, R* sS, int ML,
+                                           const EdgeRec<R>& e0) {
+  const int s0 = threadIdx.x;
+  for (int s = threadIdx.x; s < d.nedge; s += blockDim.x) {
     const EdgeRec<R> e = (s == s0) ? e0 : a.edges[d.edge0 + s];
     const EdgeView ev = edge_cells(e);
     const R vx = edge_vx(e), vy = edge_vy(e);
@@ -252,7 +264,6 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
       Rraw = ld(sW[ev.r]);
       Rr = Rraw;
       if (a.order >= 2) {
-        // limited reconstruction, kernel_LBJinterpolate gpu_solver.cu:496-549
         const int l = ev.l, r = ev.r;
         R dl[4], dr[4];
 #pragma unroll
@@ -283,13 +294,20 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
     F.dv *= e.vn;
     F.de *= e.vn;
     sF[s] = st(F);
-    sS[s] = spectral_radius(WL, Rraw, vx, vy, f.gamma);
+    sS[s] = edge_sr(e, WL, Rraw, vx, vy, f.gamma);
   }
-  __syncthreads();
+}
 
-  // 3. ordered gather and time step
+// Ordered gather of each tile cell's three edge slots, R and the local time
+// step (kernel_Flux_e2c gpu_solver.cu:768-878).  Returns this thread's
+// minimum of A*CFL/T (global time step mode).
+template <typename R>
+__device__ __forceinline__ R gather_cells(const TiledArgs<R>& a, const TileDesc& d, const FlowT<R>& f,
+                                          const SVec4<R>* sF, const R* sS, SVec4<R>* Rout, R* dtout,
+                                          const int4& sl0, R area0) {
+  const int s0 = threadIdx.x;
   R tmin = R(3.0e38);
-  for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+  for (int i = threadIdx.x; i < d.ncell; i += blockDim.x) {
     const int g = d.cell0 + i;
     const int4 sl = (i == s0) ? sl0 : a.slot[d.geo0 + g];
     const int slk[3] = {sl.x, sl.y, sl.z};
@@ -306,16 +324,80 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
         r0 -= F.d; r1 -= F.du; r2 -= F.dv; r3 -= F.de;
       }
     }
-    a.res[d.st0 + g] = SVec4<R>{r0, r1, r2, r3};
+    Rout[d.st0 + g] = SVec4<R>{r0, r1, r2, r3};
     const R area = (i == s0) ? area0 : a.cg[d.geo0 + g].area;
     if (a.global_dt) {
       const R v = area * f.cfl / T;
-      a.dt[d.st0 + g] = v;
+      dtout[d.st0 + g] = v;
       tmin = fmin(tmin, v);
     } else {
-      a.dt[d.st0 + g] = fmin(area * f.cfl / T, R(0.1));
+      dtout[d.st0 + g] = fmin(area * f.cfl / T, R(0.1));
     }
   }
+  return tmin;
+}
+
+// Fixed-order sum of one double per thread over the workgroup (wave
+// shuffles, then waves in order); the result is valid in thread 0.
+__device__ __forceinline__ double block_sum(double acc) {
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  __shared__ double wsum[16];
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) wsum[wv] = acc;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x + 63) / 64; k++) s += wsum[k];
+  return s;
+}
+
+// --------------------------------------------------------------------------
+// Kernel A: gradient + flux + gather for one tile.
+// LDS: W[nl] | grad[8][nl] | bounds[8][nl] | F[ne] | SR[ne]
+// --------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int t = tile_of_block(blockIdx.x, a.ntiles);
+  const TileDesc d = a.tiles[t];
+  if (!a.ctl[d.icase].active) return;
+  const FlowT<R> f = a.flow[d.icase];
+  const int nt = d.ncell, nl = d.ncell + d.nhalo;
+  const int ML = a.max_local, ME = a.max_edges;
+  SVec4<R>* sW = reinterpret_cast<SVec4<R>*>(smem);
+  R* sG = reinterpret_cast<R*>(sW + ML);  // [8][ML]: d/dx 4 comps, d/dy 4 comps
+  R* sB = sG + 8 * ML;                    // [8][ML]: max 4 comps, min 4 comps
+  SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
+  R* sS = reinterpret_cast<R*>(sF + ME);
+  const SVec4<R>* W = a.w_in + d.st0;
+
+  // prefetch this thread's first edge record and gather slot; their latency
+  // hides behind the gradient phase
+  const int s0 = threadIdx.x;
+  EdgeRec<R> e0{};
+  if (s0 < d.nedge) e0 = a.edges[d.edge0 + s0];
+  int4 sl0 = make_int4(0, 0, 0, 0);
+  R area0 = R(0);
+  if (s0 < nt) {
+    sl0 = a.slot[d.geo0 + d.cell0 + s0];
+    area0 = a.cg[d.geo0 + d.cell0 + s0].area;
+  }
+
+  for (int j = threadIdx.x; j < nl; j += blockDim.x) {
+    const int g = j < nt ? d.cell0 + j : a.halo[d.halo0 + j - nt];
+    const Vec4<R> Wi = ld(W[g]);
+    sW[j] = st(Wi);
+    if (a.order >= 2) {
+      const int4 nb = a.nbr[d.geo0 + g];
+      const int nbk[3] = {nb.x, nb.y, nb.z};
+      gradient_cell<R>(j, Wi, a.cg[d.geo0 + g], nbk, [&](int k) { return ld(W[nbk[k]]); }, f, sG,
+                       sB, ML);
+    }
+  }
+  __syncthreads();
+  flux_edges<R>(a, d, f, sW, sG, sB, sF, sS, ML, e0);
+  __syncthreads();
+  const R tmin = gather_cells<R>(a, d, f, sF, sS, a.res, a.dt, sl0, area0);
   if (a.global_dt) {
     __shared__ R red[512];
     red[threadIdx.x] = tmin;
@@ -411,17 +493,138 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
     }
   }
   if (a.last_stage) {
-    // fixed-order tile sum: wave shuffles, then waves in order
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    __shared__ double wsum[16];
-    const int wv = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) wsum[wv] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int k = 0; k < (int)(blockDim.x + 63) / 64; k++) s += wsum[k];
-      a.partial[t] = s;
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) a.partial[t] = s;
+  }
+}
+
+// --------------------------------------------------------------------------
+// One kernel per RK stage (see the header comment).
+// LDS: R1[MR] | R[MW] | W[MW] | grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME]
+// --------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int t = tile_of_block(blockIdx.x, a.ntiles);
+  const TileDesc d = a.tiles[t];
+  const CaseCtl& cc = a.ctl[d.icase];
+  // the first launch of a run prepares every healthy case; later ones only
+  // the cases still iterating
+  if (a.apply_update ? !cc.active : cc.status != FVM_OK) return;
+  const FlowT<R> f = a.flow[d.icase];
+  const int nt = d.ncell, NG = nt + d.nhalo, NW = NG + d.n2, NR = NW + d.n3;
+  const int MR = a.max_ring_local, MW = a.max_w_local, ML = a.max_local, ME = a.max_edges;
+  SVec4<R>* sR1 = reinterpret_cast<SVec4<R>*>(smem);
+  SVec4<R>* sR = sR1 + MR;
+  SVec4<R>* sW = sR + MW;
+  R* sG = reinterpret_cast<R*>(sW + MW);
+  R* sB = sG + 8 * ML;
+  SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
+  R* sS = reinterpret_cast<R*>(sF + ME);
+  const int cur = cc.w0_cur;
+  const SVec4<R>* W0 = (cur ? a.w0b : a.w0a) + d.st0;
+  SVec4<R>* W0n = (cur ? a.w0a : a.w0b) + d.st0;
+  const int4* lnb = a.lnb + d.lnb0;
+  auto gid = [&](int j) { return j < nt ? d.cell0 + j : a.rings[d.ring0 + j - nt]; };
+
+  const int s0 = threadIdx.x;
+  EdgeRec<R> e0{};
+  if (s0 < d.nedge) e0 = a.edges[d.edge0 + s0];
+  int4 sl0 = make_int4(0, 0, 0, 0);
+  R area0 = R(0);
+  if (s0 < nt) {
+    sl0 = a.slot[d.geo0 + d.cell0 + s0];
+    area0 = a.cg[d.geo0 + d.cell0 + s0].area;
+  }
+
+  double acc = 0.0;
+  if (a.apply_update) {
+    const SVec4<R>* Rin = a.r_in + d.st0;
+    const R eps = f.eps;
+    // 1. smoothing sweep 1 on the tile and rings 1..3 (own residual kept for
+    //    the tile and rings 1..2)
+    const int n1 = a.smoothing ? NR : NW;
+    for (int j = threadIdx.x; j < n1; j += blockDim.x) {
+      const int g = gid(j);
+      const SVec4<R> ri = Rin[g];
+      if (j < NW) sR[j] = ri;
+      if (a.smoothing) {
+        const int4 nb = a.nbr[d.geo0 + g];
+        const int nbk[3] = {nb.x, nb.y, nb.z};
+        R q0 = R(0.0), q1 = R(0.0), q2 = R(0.0), q3 = R(0.0);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          if (nbk[k] >= 0) {
+            const SVec4<R> v = Rin[nbk[k]];
+            q0 += v.d; q1 += v.du; q2 += v.dv; q3 += v.de;
+          }
+        }
+        const R den = R(1.0) + eps * R(nb.w);
+        sR1[j] = SVec4<R>{(q0 * eps + ri.d) / den, (q1 * eps + ri.du) / den,
+                          (q2 * eps + ri.dv) / den, (q3 * eps + ri.de) / den};
+      }
     }
+    __syncthreads();
+    // 2. sweep 2, selection and the RK update on the tile and rings 1..2
+    for (int j = threadIdx.x; j < NW; j += blockDim.x) {
+      const int g = gid(j);
+      SVec4<R> ri = sR[j];
+      if (a.smoothing) {
+        const int4 ln = lnb[j];
+        const int lk[3] = {ln.x, ln.y, ln.z};
+        R q0 = R(0.0), q1 = R(0.0), q2 = R(0.0), q3 = R(0.0);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          if (lk[k] >= 0) {
+            const SVec4<R> v = sR1[lk[k]];
+            q0 += v.d; q1 += v.du; q2 += v.dv; q3 += v.de;
+          }
+        }
+        const R den = R(1.0) + eps * R(a.nbr[d.geo0 + g].w);
+        const R p0 = (q0 * eps + ri.d) / den, p1 = (q1 * eps + ri.du) / den;
+        const R p2 = (q2 * eps + ri.dv) / den, p3 = (q3 * eps + ri.de) / den;
+        if (fabs(ri.d) > fabs(p0)) ri.d = p0;
+        if (fabs(ri.du) > fabs(p1)) ri.du = p1;
+        if (fabs(ri.dv) > fabs(p2)) ri.dv = p2;
+        if (fabs(ri.de) > fabs(p3)) ri.de = p3;
+      }
+      const R rk = a.rk * a.dt_in[d.st0 + g];
+      const R area = a.cg[d.geo0 + g].area;
+      const SVec4<R> w0 = W0[g];
+      SVec4<R> wn;
+      wn.d = w0.d - rk * ri.d / area;
+      wn.du = w0.du - rk * ri.du / area;
+      wn.dv = w0.dv - rk * ri.dv / area;
+      wn.de = w0.de - rk * ri.de / area;
+      sW[j] = wn;
+      if (a.close_step && j < nt) {
+        W0n[g] = wn;
+        const double df = (double)wn.d - (double)w0.d;
+        acc += df * df;
+      }
+    }
+  } else {
+    for (int j = threadIdx.x; j < NW; j += blockDim.x) sW[j] = W0[gid(j)];
+  }
+  __syncthreads();
+  // 3. gradient and bounds of the tile and ring 1 from the updated state
+  if (a.order >= 2) {
+    for (int j = threadIdx.x; j < NG; j += blockDim.x) {
+      const int4 ln = lnb[j];
+      const int nbk[3] = {ln.x, ln.y, ln.z};
+      gradient_cell<R>(j, ld(sW[j]), a.cg[d.geo0 + gid(j)], nbk,
+                       [&](int k) { return ld(sW[nbk[k]]); }, f, sG, sB, ML);
+    }
+  }
+  __syncthreads();
+  // 4. flux and spectral radius of the tile's edges
+  flux_edges<R>(a, d, f, sW, sG, sB, sF, sS, ML, e0);
+  __syncthreads();
+  // 5. ordered gather: this stage's residual and time step
+  gather_cells<R>(a, d, f, sF, sS, a.res, a.dt, sl0, area0);
+  if (a.close_step && a.apply_update) {
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) a.partial[t] = s;
   }
 }
 

@@ -22,7 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("FVM_LIBRARY", os.path.join(HERE, "libfvm_hip.so"))
 
 FVM_FP64, FVM_FP32 = 0, 1
-FVM_KERNELS_TILED, FVM_KERNELS_PLAIN = 0, 1
+FVM_KERNELS_TILED, FVM_KERNELS_PLAIN, FVM_KERNELS_SPLIT = 0, 1, 2
 
 STATUS = {0: "FVM_OK", 1: "FVM_ERR_ARGUMENT", 2: "FVM_ERR_MESH", 3: "FVM_ERR_AREA",
           4: "FVM_ERR_HIP", 5: "FVM_ERR_NAN", 6: "FVM_ERR_NOT_CONVERGED",
@@ -147,7 +147,8 @@ def scheme_opts(order=2, smoothing=True, eps_smooth=0.8, global_dt=False, precis
     o.eps_smooth = eps_smooth
     o.global_dt = int(bool(global_dt))
     o.precision = {"fp64": FVM_FP64, "fp32": FVM_FP32}[precision]
-    o.kernels = {"tiled": FVM_KERNELS_TILED, "plain": FVM_KERNELS_PLAIN}[kernels]
+    o.kernels = {"tiled": FVM_KERNELS_TILED, "plain": FVM_KERNELS_PLAIN,
+                 "split": FVM_KERNELS_SPLIT}[kernels]
     o.tile_cells = tile_cells
     o.threads = threads
     o.steps_per_graph = steps_per_graph

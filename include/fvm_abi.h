@@ -66,9 +66,12 @@ typedef enum fvm_precision {
 } fvm_precision;
 
 typedef enum fvm_kernels {
-    FVM_KERNELS_TILED = 0, /* two kernels per RK stage over LDS-staged cell tiles (default) */
-    FVM_KERNELS_PLAIN = 1  /* the reference's kernel structure: one kernel per phase, edge
+    FVM_KERNELS_TILED = 0, /* one kernel per RK stage over LDS-staged cell tiles (default); with
+                              the global time step, two kernels per stage */
+    FVM_KERNELS_PLAIN = 1, /* the reference's kernel structure: one kernel per phase, edge
                               slots in global memory (FP64 only) */
+    FVM_KERNELS_SPLIT = 2  /* two kernels per RK stage (flux, then smoothing + update) over the
+                              same tiles */
 } fvm_kernels;
 
 /* Scheme options.  The reference hard-codes second order with the limiter,
@@ -161,10 +164,11 @@ fvm_status fvm_batch_get_state(fvm_batch *b, int32_t icase, double *rho, double 
 fvm_status fvm_batch_get_forces(fvm_batch *b, double *cl, double *cd); /* [ncase] each */
 fvm_status fvm_batch_get_timing(fvm_batch *b, double *ms, int64_t *cell_stage_updates);
 /* Average device time of one launch of each tiled kernel of an RK stage, from
- * HIP events on the batch's stream around `reps` back-to-back launches:
- * ms[0] = flux_tile (gradient + flux + gather), ms[1] = update_tile
- * (smoothing + RK update).  The launches use the current state and leave it
- * unchanged.  No counterpart in the reference (its per-phase wall-clock timers,
+ * HIP events on the batch's stream around `reps` back-to-back launches.  One
+ * kernel per stage: ms[0] = stage_tile (smoothing + update + gradient + flux
+ * + gather), ms[1] = 0.  Two kernels per stage: ms[0] = flux_tile (gradient
+ * + flux + gather), ms[1] = update_tile (smoothing + RK update).  The
+ * launches use the current state and leave it unchanged.  No counterpart in the reference (its per-phase wall-clock timers,
  * gpu_solver.cu:2214-2423, play this role).  Tiled kernels only. */
 fvm_status fvm_batch_kernel_times(fvm_batch *b, int32_t reps, double ms[2]);
 fvm_status fvm_kernel_times(fvm_solver *s, int32_t reps, double ms[2]);

@@ -18,7 +18,8 @@ FLOWS = [dict(mach=0.5, alpha_deg=0.0, cfl=1.5), dict(mach=0.8, alpha_deg=1.25, 
          dict(mach=0.7, alpha_deg=3.0, cfl=1.2)]
 
 
-@pytest.mark.parametrize("precision,kernels", [("fp64", "tiled"), ("fp32", "tiled"), ("fp64", "plain")])
+@pytest.mark.parametrize("precision,kernels", [("fp64", "tiled"), ("fp32", "tiled"), ("fp64", "plain"),
+                                               ("fp32", "split")])
 def test_batch_equals_single_solves(meshes, precision, kernels):
     case_mesh = [0, 1, 1, 0, 1]
     flows = [FLOWS[i % 3] for i in range(len(case_mesh))]
@@ -100,7 +101,9 @@ def test_kernel_times_and_config(meshes):
     s = Solver(meshes[1], precision="fp32", tile_cells=64, threads=128, **FLOWS[1])
     s.run(3)
     ms = s.kernel_times(reps=5)
-    assert all(0 < v < 10 for v in ms)
+    assert 0 < ms[0] < 10 and ms[1] == 0.0          # one kernel per stage
+    ms2 = Solver(meshes[1], precision="fp32", kernels="split", **FLOWS[1]).kernel_times(reps=5)
+    assert all(0 < v < 10 for v in ms2)             # flux_tile, update_tile
     cfg = s.config()
     assert cfg["tile_cells"] == 64 and cfg["threads"] == 128 and cfg["tiles"] >= meshes[1].ncell // 64
     W = s.state()

@@ -49,7 +49,7 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("kernels", ["plain", "tiled"])
+@pytest.mark.parametrize("kernels", ["plain", "tiled", "split"])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"M{c[0]}-a{c[1]}-o{c[3]}-s{int(c[4])}-g{int(c[5])}")
 def test_fp64_bitwise_vs_oracle(rae3k, kernels, case):
     mach, alpha, cfl, order, smooth, gdt = case
@@ -74,11 +74,22 @@ def test_fp64_bitwise_vs_oracle(rae3k, kernels, case):
 def test_tiled_fp64_matches_oracle_naca(naca3k):
     steps = 60
     orc, hist_o = oracle_run(naca3k, steps, 0.8, 1.25, 1.5, order=2, smoothing=True)
-    for tile in (32, 160):
-        s = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp64", kernels="tiled", tile_cells=tile)
+    for kernels, tile in (("tiled", 32), ("tiled", 160), ("split", 48)):
+        s = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp64", kernels=kernels, tile_cells=tile)
         hist, n, st = s.run(steps)
         assert st == 0 and n == steps
         assert np.array_equal(s.state(), orc.state())
+
+
+def test_fp32_one_and_two_kernel_paths_agree(naca3k):
+    # the one-kernel-per-stage path recomputes ring cells redundantly; it must
+    # give the same FP32 bits as the two-kernel path
+    a = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp32", kernels="tiled")
+    b = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp32", kernels="split")
+    ha, _, _ = a.run(50)
+    hb, _, _ = b.run(50)
+    assert np.array_equal(a.state(), b.state())
+    np.testing.assert_array_equal(ha, hb)
 
 
 def test_fp32_first_order_converged_vs_oracle(naca3k):

@@ -60,7 +60,7 @@ def test_driver_reproduces_reference_residual_file(driver, tmp_path):
     # node averages and wall pressure files are written and finite
     res = open(tmp_path / "unresult.plt").read().splitlines()
     assert res[3].startswith("N=1907,E=3558")
-    vals = np.array([[float(v) for v in l.split()] for l in res[5:5 + 1907]])
+    vals = np.array([[float(v) for v in l.split()] for l in res[6:6 + 1907]])
     assert np.isfinite(vals).all() and (vals[:, 2] > 0).all()
     cp = open(tmp_path / "unairfoilpress.plt").read().splitlines()
     assert len(cp) == 3 + 2 * 128 + 128
