@@ -16,7 +16,10 @@ Measured in one run:
                    algorithmic bytes per launch / average launch time from HIP
                    events on the solver stream
   accuracy         FP32 vs FP64 (the FP64 path equals the FP64 oracle bit for
-                   bit) after --acc-steps steps: |dCl|, |dCd|, density rel. L2
+                   bit) on the workload's limit cycle: window means of Cl/Cd,
+                   beside FP64 vs FP64 from a 1e-12-perturbed start
+  accuracy_converged   FP32 vs FP64 on a converging first-order case
+  batch64          64 cases on 4 uneven meshes in one batch (FP32 Mcsu/s)
   cpu_baseline     the FP64 CPU oracle on this host, one thread, on a bounded
                    sample of steps of the same mesh
 """
@@ -91,6 +94,103 @@ def cpu_baseline(mesh, steps):
                        f"(gcc -O2, one thread), {sec:.1f} s")
 
 
+def limit_cycle_accuracy(mesh, flow, t0, t1, every):
+    """FP32 against FP64 on the workload itself.  With the limiter the
+    second-order transonic solution does not converge: the residual stalls
+    near 1e-2 and Cl keeps cycling by a few 1e-4, and two FP64 runs whose
+    start states differ by 1e-12 part on that cycle just as FP32 and FP64 do.
+    So the coefficients are compared as means over [t0, t1] (sampled every
+    `every` steps), beside the same statistics for FP64 against FP64 from a
+    perturbed start, which measures the cycle's own sensitivity."""
+    from gpueularsolver2d_amd import Solver
+    A = Solver(mesh, precision="fp64", **flow)
+    B = Solver(mesh, precision="fp64", **flow)
+    W = B.state()
+    B.set_state(W * (1.0 + 1e-12 * np.random.default_rng(0).standard_normal(W.shape)))
+    F = Solver(mesh, precision="fp32", **flow)
+    for s in (A, B, F):
+        s.run(t0)
+    smp, resd = [], []
+    for n in range(t0 + every, t1 + 1, every):
+        row = []
+        for s in (A, B, F):
+            h, _, _ = s.run(n)
+            row += list(s.forces())
+            if s is A:
+                resd.append(h[n])
+        smp.append(row)
+    smp = np.array(smp)  # cl_A cd_A cl_B cd_B cl_F cd_F
+    WA, WF = A.state(), F.state()
+    stats = lambda i: dict(dcl_mean=float(smp[:, i].mean() - smp[:, 0].mean()),
+                           dcd_mean=float(smp[:, i + 1].mean() - smp[:, 1].mean()),
+                           dcl_rms=float(np.sqrt(np.mean((smp[:, i] - smp[:, 0]) ** 2))),
+                           dcl_max=float(np.abs(smp[:, i] - smp[:, 0]).max()))
+    return dict(window=[t0, t1], every=every, samples=len(smp),
+                cl_fp64_mean=float(smp[:, 0].mean()), cd_fp64_mean=float(smp[:, 1].mean()),
+                cl_fp64_std=float(smp[:, 0].std()),
+                resd_fp64_window=[float(min(resd)), float(max(resd))],
+                fp32_vs_fp64=dict(stats(4), rho_rel_l2_end=float(
+                    np.linalg.norm(WF[:, 0] - WA[:, 0]) / np.linalg.norm(WA[:, 0]))),
+                fp64_perturbed_vs_fp64=stats(2))
+
+
+def converged_accuracy(mesh, max_steps, epsl=1e-7):
+    """FP32 against FP64 on a case that converges: first order, M 0.5,
+    alpha 1.25, no smoothing, same mesh, run until the FP64 residual drops
+    below epsl (both precisions run the same number of steps)."""
+    from gpueularsolver2d_amd import Solver
+    flow = dict(mach=0.5, alpha_deg=1.25, cfl=1.5, order=1, smoothing=False)
+    A = Solver(mesh, precision="fp64", **flow)
+    h, n, _ = A.run(max_steps, epsl=epsl)
+    F = Solver(mesh, precision="fp32", **flow)
+    hF, _, _ = F.run(n)
+    (cl, cd), (clF, cdF) = A.forces(), F.forces()
+    WA, WF = A.state(), F.state()
+    return dict(steps=int(n), resd_fp64=float(h[-1]), resd_fp32=float(hF[-1]), cl_fp64=cl, cd_fp64=cd,
+                dcl=float(clF - cl), dcd=float(cdF - cd),
+                rho_rel_l2=float(np.linalg.norm(WF[:, 0] - WA[:, 0]) / np.linalg.norm(WA[:, 0])))
+
+
+def batch64(steps):
+    """64 cases in one batch: four NACA 0012 meshes of uneven size (~3k, 10k,
+    50k, 90k triangles, seeds 11-14) x 16 (Mach, alpha) pairs on the grid
+    M in {0.5, 0.6167, 0.7333, 0.85} x alpha in {0, 1, 2, 3} deg; second order
+    with the limiter and smoothing, CFL 1.5.  FP32 throughput over `steps`
+    steps, and the per-case |dCl| against the same batch in FP64 after them
+    (instantaneous values; see the limit-cycle comparison for their meaning).
+    Cases that diverge (non-finite residual, in FP64 as well) stop at that
+    step; only the cell-stage updates done count."""
+    from gpueularsolver2d_amd import Batch, naca0012_mesh
+    meshes = [naca0012_mesh(n, seed=s) for n, s in ((3000, 11), (10000, 12), (50000, 13), (90000, 14))]
+    flows = [dict(mach=float(m), alpha_deg=float(a), cfl=1.5)
+             for m in np.linspace(0.5, 0.85, 4) for a in (0.0, 1.0, 2.0, 3.0)]
+    case_mesh = [i % 4 for i in range(64)]
+    case_flow = [flows[(i // 4) % 16] for i in range(64)]
+    cells = sum(meshes[m].ncell for m in case_mesh)
+    F = Batch(meshes, case_mesh, case_flow, precision="fp32")
+    F.run(20)
+    sync()
+    t0 = time.perf_counter()
+    _, nF, stF = F.run(20 + steps, check=False)
+    sync()
+    t = time.perf_counter() - t0
+    A = Batch(meshes, case_mesh, case_flow, precision="fp64")
+    _, nA, stA = A.run(20 + steps, check=False)
+    # a case whose residual turns non-finite stops there; the scheme itself
+    # (FP64, equal to the oracle) diverges for such cases too
+    done = sum((int(nF[i]) - 20) * meshes[m].ncell for i, m in enumerate(case_mesh))
+    ok = (stF == 0) & (stA == 0)
+    (clF, cdF), (clA, cdA) = F.forces(), A.forces()
+    d = np.abs(clF - clA)[ok]
+    return dict(value=round(4.0 * done / t / 1e6, 1), unit="Mcsu/s", cases=64, cells=int(cells),
+                mesh_cells=[m.ncell for m in meshes], steps=steps, launch=F.config(),
+                diverged={str(i): dict(mesh_cells=meshes[case_mesh[i]].ncell, **case_flow[i],
+                                       fp32_step=int(nF[i]), fp64_step=int(nA[i]))
+                          for i in range(64) if not ok[i]},
+                dcl_max=float(d.max()), dcl_median=float(np.median(d)),
+                dcd_max=float(np.abs(cdF - cdA)[ok].max()))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,7 +198,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cells", type=int, default=200000)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--acc-steps", type=int, default=5000)
+    ap.add_argument("--acc-window", type=lambda v: [int(x) for x in v.split(":")],
+                    default=[30000, 70000, 100],
+                    help="start:end:every of the averaging window of the limit-cycle comparison "
+                         "(end 0 skips it)")
+    ap.add_argument("--conv-steps", type=int, default=40000,
+                    help="step limit of the converged first-order comparison (0 skips it)")
+    ap.add_argument("--batch-steps", type=int, default=2000,
+                    help="steps of the 64-case batch (0 skips it)")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-cells", type=int, default=0)
@@ -149,31 +256,9 @@ def main():
     v64 = csu / t64 / 1e6
     vpl = 4.0 * C * plain_steps / tpl / 1e6
 
-    # accuracy after acc_steps from the same start.  With the limiter the
-    # second-order scheme settles into a small limit cycle (the residual
-    # stalls near 1e-2..1e-3), so besides the instantaneous difference the
-    # coefficients are also averaged over the last `window` steps.
-    acc = None
-    if args.acc_steps > 0:
-        n = max(args.acc_steps, args.warmup + args.steps)
-        window, every = 1000, 50
-        start = max(n - window, args.warmup + args.steps)
-        s32.run(start)
-        s64.run(start)
-        samples = []
-        for k in range(start + every, n + 1, every):
-            s32.run(k)
-            s64.run(k)
-            samples.append(s32.forces() + s64.forces())
-        smp = np.array(samples)
-        W32, W64 = s32.state(), s64.state()
-        cl32, cd32, cl64, cd64 = smp[-1]
-        acc = dict(steps=n, cl_fp64=cl64, cd_fp64=cd64, dcl=abs(cl32 - cl64), dcd=abs(cd32 - cd64),
-                   rho_rel_l2=float(np.linalg.norm(W32[:, 0] - W64[:, 0]) / np.linalg.norm(W64[:, 0])),
-                   window=[int(start), int(n)],
-                   dcl_mean=float(abs(smp[:, 0].mean() - smp[:, 2].mean())),
-                   dcd_mean=float(abs(smp[:, 1].mean() - smp[:, 3].mean())),
-                   cl_fp64_range=float(np.ptp(smp[:, 2])))
+    acc = limit_cycle_accuracy(mesh, flow, *args.acc_window) if args.acc_window[1] > 0 else None
+    conv = converged_accuracy(mesh, args.conv_steps) if args.conv_steps > 0 else None
+    b64 = batch64(args.batch_steps) if args.batch_steps > 0 else None
 
     w = 4  # FP32 state and stored geometry
     if kt[1] > 0:   # two kernels per stage: flux_tile dominates
@@ -207,6 +292,8 @@ def main():
         "fp64_plain": {"value": round(vpl, 1), "steps": plain_steps},
         "fp32_over_fp64_plain": round(v32 / vpl, 2),
         "accuracy": acc,
+        "accuracy_converged": conv,
+        "batch64": b64,
     }
     print(json.dumps(out))
     return 0

@@ -39,15 +39,70 @@ static const double kRK[4] = {0.0833, 0.2069, 0.4265, 1.0};  // gpu_fvm.cu:10
 // ---------------------------------------------------------------------------
 // finalize_step: per case residual and stop test
 // ---------------------------------------------------------------------------
-// Closes the time step of every active case after the last stage (one
-// workgroup of 64 per case).  A separate launch: the kernel boundary orders
-// the tiles' partial sums before the close without a device-scope fence,
-// which on MI355X would write back a whole L2 per workgroup.
-__global__ void __launch_bounds__(64) finalize_kernel(CaseCtl* ctl, const double* partial,
-                                                      double* hist, int hist_stride, int flip_w0) {
+// Adds the compensated pair (s2, c2) to (s, c): the exact rounding error of
+// s + s2 (Knuth's two-sum) joins the compensation.
+__device__ __forceinline__ void add_pair(double& s, double& c, double s2, double c2) {
+  const double t = s + s2;
+  const double bb = t - s;
+  const double e = (s - (t - bb)) + (s2 - bb);
+  c = (c + c2) + e;
+  s = t;
+}
+
+// Closes the time step of every active case after the last stage, one
+// workgroup of kFinalizeThreads per case: the FP64 residual from the tiles'
+// partial sums of (rho - rho0)^2, Resd = sqrt((drou + 1e-32) / drou_of_step_1),
+// the history entry and the stop test step < max_steps && Resd > epsl
+// (gpu_solver.cu:2200, :2403-2433).  The sum is compensated and its order is
+// fixed: thread t adds partials t, t + 256, ... (Neumaier), then a butterfly
+// over the wave combines lane pairs lower lane first (so both lanes of a pair
+// hold bitwise the same result), then the four waves in order.
+// A separate launch: the kernel boundary orders the tiles' partial sums
+// before the close without a device-scope fence, which on MI355X would write
+// back a whole L2 per workgroup.
+// flip_w0: the step's new state went to the other step-start buffer.
+constexpr int kFinalizeThreads = 256;
+
+__global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(CaseCtl* ctl, const double* partial,
+                                                                    double* hist, int hist_stride,
+                                                                    int flip_w0) {
   CaseCtl& c = ctl[blockIdx.x];
   if (!c.active) return;
-  close_step(c, partial, hist, hist_stride, blockIdx.x, flip_w0);
+  const int n = c.ntile;
+  const double* p = partial + c.tile0;
+  double s = 0.0, cc = 0.0;
+  for (int k = threadIdx.x; k < n; k += kFinalizeThreads) neumaier(s, cc, p[k]);
+  const int lane = threadIdx.x & 63;
+  for (int off = 1; off < 64; off <<= 1) {
+    const double s2 = __shfl_xor(s, off, 64), c2 = __shfl_xor(cc, off, 64);
+    if (lane & off) {  // this lane is the upper one of the pair
+      double ls = s2, lc = c2;
+      add_pair(ls, lc, s, cc);
+      s = ls;
+      cc = lc;
+    } else {
+      add_pair(s, cc, s2, c2);
+    }
+  }
+  __shared__ double ws[kFinalizeThreads / 64], wc[kFinalizeThreads / 64];
+  if (lane == 0) {
+    ws[threadIdx.x >> 6] = s;
+    wc[threadIdx.x >> 6] = cc;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double S = ws[0], Cc = wc[0];
+  for (int k = 1; k < kFinalizeThreads / 64; k++) add_pair(S, Cc, ws[k], wc[k]);
+  const double drou = S + Cc;
+  const int step = c.step + 1;
+  c.step = step;
+  if (flip_w0) c.w0_cur ^= 1;
+  if (step == 1) c.epsm = drou;
+  const double resd = sqrt((drou + 1.0E-32) / c.epsm);
+  c.resd = resd;
+  if (step < hist_stride) hist[(size_t)blockIdx.x * hist_stride + step] = resd;
+  if (!isfinite(resd)) c.status = FVM_ERR_NAN;
+  if (!(step < c.max_steps && resd > c.epsl)) c.active = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -163,16 +218,22 @@ class Batch {
   int64_t last_csu = 0;
   int ncase() const { return (int)cases_.size(); }
   int ncell(int icase) const { return topo_[cases_[icase].mesh].ncell; }
-  void config(int32_t cfg[4]) const {
+  void config(int32_t cfg[6]) const {
     cfg[0] = tile_cells_;
     cfg[1] = threads_;
     cfg[2] = steps_per_graph_;
     cfg[3] = total_tiles_;
+    cfg[4] = blocks_per_cu_;
+    cfg[5] = cus_;
   }
 
  private:
   template <typename R>
   void setup_tiled();
+  template <typename R>
+  size_t tile_lds() const;
+  template <typename R>
+  int auto_tile_cells();
   void setup_plain();
   template <typename R>
   TiledArgs<R> tiled_args();
@@ -198,6 +259,7 @@ class Batch {
   std::vector<CaseInfo> cases_;
   Arena arena_;
   int tile_cells_ = 0, threads_ = 256, steps_per_graph_ = 20, total_tiles_ = 0;
+  int cus_ = 0, blocks_per_cu_ = 0;  // CUs of the device; workgroups of the tile kernel per CU
   size_t lds_a_ = 0, lds_b_ = 0, lds_s_ = 0;
   bool fused_ = false;      // one kernel per RK stage
   bool r0_valid_ = false;   // (fused) the residual of stage 0 matches the current state
@@ -289,16 +351,6 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
     ci.f64 = make_flow(flows[i], o.eps_smooth, &ci.rhoE_inf);
     cases_.push_back(ci);
   }
-  fused_ = o.kernels == FVM_KERNELS_TILED && !o.global_dt;
-  if (o.kernels != FVM_KERNELS_PLAIN) {
-    // defaults from the launch-shape sweep on MI355X (autotune.py): 128-cell
-    // tiles with 256 threads give one pass per phase for both precisions
-    tile_cells_ = o.tile_cells > 0 ? o.tile_cells : 128;
-    if (tile_cells_ > 8192)
-      throw Error{FVM_ERR_ARGUMENT, "opts.tile_cells must be at most 8192"};
-    threads_ = o.threads > 0 ? o.threads : 256;
-    for (int m = 0; m < nmesh; m++) tiled_.push_back(build_tiled(topo_[m], tile_cells_));
-  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     throw Error{FVM_ERR_NO_DEVICE, "no HIP device available"};
@@ -306,6 +358,19 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
     throw Error{FVM_ERR_NO_DEVICE, "device ordinal " + std::to_string(o.device) + " out of range"};
   device_ = o.device;
   HIP_OK(hipSetDevice(device_));
+  HIP_OK(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device_));
+  fused_ = o.kernels == FVM_KERNELS_TILED && !o.global_dt;
+  if (o.kernels != FVM_KERNELS_PLAIN) {
+    if (o.tile_cells > 8192)
+      throw Error{FVM_ERR_ARGUMENT, "opts.tile_cells must be at most 8192"};
+    threads_ = o.threads > 0 ? o.threads : 256;
+    if (o.tile_cells > 0)
+      tile_cells_ = o.tile_cells;
+    else
+      tile_cells_ = o.precision == FVM_FP64 ? auto_tile_cells<double>() : auto_tile_cells<float>();
+    tiled_.clear();
+    for (int m = 0; m < nmesh; m++) tiled_.push_back(build_tiled(topo_[m], tile_cells_));
+  }
   HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HIP_OK(hipEventCreate(&ev0_));
   HIP_OK(hipEventCreate(&ev1_));
@@ -328,6 +393,59 @@ Batch::~Batch() {
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+// LDS per workgroup of the kernels this batch launches, for its tile tables.
+template <typename R>
+size_t Batch::tile_lds() const {
+  int ML = 0, ME = 0, MR = 0, MW = 0;
+  for (auto& t : tiled_) {
+    ML = std::max(ML, t.max_local);
+    ME = std::max(ME, t.max_edges);
+    MR = std::max(MR, t.max_ring_local);
+    MW = std::max(MW, t.max_w_local);
+  }
+  return fused_ ? stage_tile_lds<R>(MR, MW, ML, ME)
+                : std::max(flux_tile_lds<R>(ML, ME), update_tile_lds<R>(ML));
+}
+
+// Tile size when the caller leaves it to the library.  Workgroups that
+// outnumber the CUs' free slots by a fraction of a wave leave a tail, and
+// fewer workgroups than slots are not spread evenly over the CUs, so the
+// tile size is chosen to make the batch's tiles fill whole waves of slots:
+// the smallest tile in [kMinTile, kMaxTile] whose tile count fits in the
+// number of waves that kMaxTile-cell tiles need.  The slots per CU come from
+// the occupancy of the tile kernel at kMaxTile (registers and LDS), which
+// smaller tiles can only raise.
+template <typename R>
+int Batch::auto_tile_cells() {
+  constexpr int kMinTile = 64, kMaxTile = 160;
+  const int nmesh = (int)topo_.size();
+  tiled_.clear();
+  for (int m = 0; m < nmesh; m++) tiled_.push_back(build_tiled(topo_[m], kMaxTile));
+  const size_t lds = tile_lds<R>();
+  int per_cu = 0;
+  if (lds <= 160 * 1024) {
+    HIP_OK(set_tile_lds_limit<R>(lds));
+    HIP_OK(tile_blocks_per_cu<R>(fused_ ? kStageTile : kFluxTile, threads_, lds, &per_cu));
+  }
+  if (per_cu < 1) return kMaxTile;
+  auto ntiles = [&](int t) {
+    long n = 0;
+    for (auto& c : cases_) n += (topo_[c.mesh].ncell + t - 1) / t;
+    return n;
+  };
+  const long slots = (long)per_cu * cus_;
+  const long waves = (ntiles(kMaxTile) + slots - 1) / slots;
+  int lo = kMinTile, hi = kMaxTile;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (ntiles(mid) <= waves * slots)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
 template <typename R>
 void Batch::setup_tiled() {
   const int ncase = (int)cases_.size(), nmesh = (int)topo_.size();
@@ -348,6 +466,8 @@ void Batch::setup_tiled() {
     throw Error{FVM_ERR_ARGUMENT, "tile_cells too large: the tile kernels need " +
                                       std::to_string(lds_max) + " bytes of LDS"};
   HIP_OK(set_tile_lds_limit<R>(std::max(lds_max, lds_b_)));
+  HIP_OK(tile_blocks_per_cu<R>(fused_ ? kStageTile : kFluxTile, threads_, fused_ ? lds_s_ : lds_a_,
+                               &blocks_per_cu_));
   // geometry offsets per mesh
   std::vector<int> geo0(nmesh), halo0(nmesh), edge0(nmesh), wall0(nmesh), ring0(nmesh), lnb0(nmesh);
   int ncell_all = 0, nhalo_all = 0, nedge_all = 0, nwall_all = 0, nring_all = 0, nlnb_all = 0;
@@ -748,7 +868,7 @@ void Batch::enqueue_fused_steps(int nsteps) {
       a.close_step = k == 4;
       HIP_OK(launch_stage_tile<R>(a, threads_, lds_s_, stream_));
     }
-    hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(64), 0, stream_, d_ctl_,
+    hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(kFinalizeThreads), 0, stream_, d_ctl_,
                        d_partial_, d_hist_, hist_stride_, 1);
     HIP_OK(hipGetLastError());
   }
@@ -781,7 +901,7 @@ void Batch::enqueue_tiled_steps(int nsteps) {
                                        static_cast<R*>(d_dtg_), (int)cases_.size(), stream_));
       HIP_OK(launch_update_tile<R>(a, threads_, lds_b_, stream_));
     }
-    hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(64), 0, stream_, d_ctl_,
+    hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(kFinalizeThreads), 0, stream_, d_ctl_,
                        d_partial_, d_hist_, hist_stride_, 0);
     HIP_OK(hipGetLastError());
   }
@@ -848,7 +968,7 @@ void Batch::enqueue_plain_steps(int nsteps) {
       plain_launch_step(plain_[c], d_ctl_, c, d_flow64_, o_.order, o_.smoothing, o_.global_dt,
                         stream_);
     HIP_OK(hipGetLastError());
-    hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(64), 0, stream_, d_ctl_,
+    hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(kFinalizeThreads), 0, stream_, d_ctl_,
                        d_partial_, d_hist_, hist_stride_, 0);
     HIP_OK(hipGetLastError());
   }
@@ -1195,14 +1315,14 @@ fvm_status fvm_batch_get_timing(fvm_batch* b, double* ms, int64_t* csu) {
   });
 }
 
-fvm_status fvm_batch_get_config(fvm_batch* b, int32_t config[4]) {
+fvm_status fvm_batch_get_config(fvm_batch* b, int32_t config[6]) {
   return guard([&] {
     if (!b || !config) throw Error{FVM_ERR_ARGUMENT, "NULL argument"};
     b->b->config(config);
   });
 }
 
-fvm_status fvm_get_config(fvm_solver* s, int32_t config[4]) {
+fvm_status fvm_get_config(fvm_solver* s, int32_t config[6]) {
   return guard([&] {
     if (!s || !config) throw Error{FVM_ERR_ARGUMENT, "NULL argument"};
     s->b->config(config);

@@ -20,6 +20,11 @@ hipError_t launch_global_dt_min(const CaseCtl* ctl, const R* tile_min, R* dt_glo
 // Largest dynamic LDS the two tile kernels may use (bytes).
 template <typename R>
 hipError_t set_tile_lds_limit(size_t bytes);
+// Workgroups of one tile kernel that a CU holds at once with this workgroup
+// size and dynamic LDS (registers and LDS both counted).
+enum TileKernel { kFluxTile = 0, kUpdateTile = 1, kStageTile = 2 };
+template <typename R>
+hipError_t tile_blocks_per_cu(TileKernel k, int threads, size_t lds, int* blocks);
 
 template <typename R>
 inline size_t flux_tile_lds(int max_local, int max_edges) {
@@ -32,8 +37,10 @@ inline size_t update_tile_lds(int max_local) {
 }
 template <typename R>
 inline size_t stage_tile_lds(int max_ring_local, int max_w_local, int max_local, int max_edges) {
-  return (size_t)(max_ring_local + 2 * max_w_local) * sizeof(SVec4<R>) +
-         flux_tile_lds<R>(max_local, max_edges) - (size_t)max_local * sizeof(SVec4<R>);
+  // W[MW] | union { R1[MR] | R[MW] ; grad + bounds + F + SR }
+  const size_t res = (size_t)(max_ring_local + max_w_local) * sizeof(SVec4<R>);
+  const size_t flux = flux_tile_lds<R>(max_local, max_edges) - (size_t)max_local * sizeof(SVec4<R>);
+  return (size_t)max_w_local * sizeof(SVec4<R>) + (res > flux ? res : flux);
 }
 
 }  // namespace fvm

@@ -171,37 +171,6 @@ __device__ __forceinline__ void neumaier(double& s, double& c, double v) {
   s = t;
 }
 
-// Closes one time step of case c from its per-tile (or per-block) partial
-// sums of (rho - rho0)^2: fixed-order compensated sum over 64 lanes, then the
-// normalised residual Resd = sqrt((drou + 1e-32) / drou_of_step_1), the
-// history entry and the stop test step < max_steps && Resd > epsl
-// (gpu_solver.cu:2200, :2403-2433).  Called by one full 64-lane wave.
-// flip_w0: the step's new state went to the other step-start buffer.
-__device__ inline void close_step(CaseCtl& c, const double* partial, double* hist, int hist_stride,
-                                  int icase, int flip_w0) {
-  const int lane = threadIdx.x & 63;
-  const int n = c.ntile, per = (n + 63) / 64;
-  const int b = lane * per, e = min(n, b + per);
-  double s = 0.0, cc = 0.0;
-  for (int k = b; k < e; k++) neumaier(s, cc, partial[c.tile0 + k]);
-  double v = s + cc;
-  // lane-ordered combination (lane 0 first), the same on every run
-  double tot = 0.0, ctot = 0.0;
-  for (int k = 0; k < 64; k++) neumaier(tot, ctot, __shfl(v, k, 64));
-  const double drou = tot + ctot;
-  if (lane == 0) {
-    const int step = c.step + 1;
-    c.step = step;
-    if (flip_w0) c.w0_cur ^= 1;
-    if (step == 1) c.epsm = drou;
-    const double resd = sqrt((drou + 1.0E-32) / c.epsm);
-    c.resd = resd;
-    if (step < hist_stride) hist[(size_t)icase * hist_stride + step] = resd;
-    if (!isfinite(resd)) c.status = FVM_ERR_NAN;
-    if (!(step < c.max_steps && resd > c.epsl)) c.active = 0;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Phases shared by flux_tile and stage_tile
 // ---------------------------------------------------------------------------
@@ -500,7 +469,10 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
 
 // --------------------------------------------------------------------------
 // One kernel per RK stage (see the header comment).
-// LDS: R1[MR] | R[MW] | W[MW] | grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME]
+// LDS: W[MW] | union { R1[MR] | R[MW] ;  grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME] }
+// The residual buffers of steps 1-2 are dead once W is complete, so the
+// gradient and flux buffers of steps 3-5 reuse their space (a smaller
+// footprint lets more workgroups share a CU).
 // --------------------------------------------------------------------------
 template <typename R>
 __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
@@ -514,9 +486,9 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   const FlowT<R> f = a.flow[d.icase];
   const int nt = d.ncell, NG = nt + d.nhalo, NW = NG + d.n2, NR = NW + d.n3;
   const int MR = a.max_ring_local, MW = a.max_w_local, ML = a.max_local, ME = a.max_edges;
-  SVec4<R>* sR1 = reinterpret_cast<SVec4<R>*>(smem);
+  SVec4<R>* sW = reinterpret_cast<SVec4<R>*>(smem);
+  SVec4<R>* sR1 = sW + MW;
   SVec4<R>* sR = sR1 + MR;
-  SVec4<R>* sW = sR + MW;
   R* sG = reinterpret_cast<R*>(sW + MW);
   R* sB = sG + 8 * ML;
   SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);

@@ -234,10 +234,10 @@ class Solver:
         return ms.value, csu.value
 
     def config(self):
-        cfg = np.zeros(4, np.int32)
+        cfg = np.zeros(6, np.int32)
         _check(load_library().fvm_get_config(self.h, _iptr(cfg)))
         return dict(tile_cells=int(cfg[0]), threads=int(cfg[1]), steps_per_graph=int(cfg[2]),
-                    tiles=int(cfg[3]))
+                    tiles=int(cfg[3]), blocks_per_cu=int(cfg[4]), cus=int(cfg[5]))
 
     def kernel_times(self, reps=20):
         """Average ms per launch of (flux_tile, update_tile)."""
@@ -304,7 +304,7 @@ class Batch:
         return float(ms[0]), float(ms[1])
 
     def config(self):
-        cfg = np.zeros(4, np.int32)
+        cfg = np.zeros(6, np.int32)
         _check(load_library().fvm_batch_get_config(self.h, _iptr(cfg)))
         return dict(tile_cells=int(cfg[0]), threads=int(cfg[1]), steps_per_graph=int(cfg[2]),
-                    tiles=int(cfg[3]))
+                    tiles=int(cfg[3]), blocks_per_cu=int(cfg[4]), cus=int(cfg[5]))

@@ -84,8 +84,8 @@ typedef struct fvm_scheme_opts {
     int32_t global_dt;      /* 0: local time step; 1: global minimum time step */
     int32_t precision;      /* fvm_precision */
     int32_t kernels;        /* fvm_kernels */
-    int32_t tile_cells;     /* cells per workgroup tile, 0 = autotuned */
-    int32_t threads;        /* threads per workgroup, 0 = autotuned */
+    int32_t tile_cells;     /* cells per workgroup tile, 0 = chosen by the library */
+    int32_t threads;        /* threads per workgroup, 0 = 256 */
     int32_t steps_per_graph;/* time steps per captured hipGraph, 0 = automatic */
     int32_t device;         /* HIP device ordinal */
 } fvm_scheme_opts;
@@ -172,9 +172,11 @@ fvm_status fvm_batch_get_timing(fvm_batch *b, double *ms, int64_t *cell_stage_up
  * gpu_solver.cu:2214-2423, play this role).  Tiled kernels only. */
 fvm_status fvm_batch_kernel_times(fvm_batch *b, int32_t reps, double ms[2]);
 fvm_status fvm_kernel_times(fvm_solver *s, int32_t reps, double ms[2]);
-/* Launch shape in use: {tile_cells, threads, steps_per_graph, total tiles}. */
-fvm_status fvm_batch_get_config(fvm_batch *b, int32_t config[4]);
-fvm_status fvm_get_config(fvm_solver *s, int32_t config[4]);
+/* Launch shape in use: {tile_cells, threads, steps_per_graph, total tiles,
+ * workgroups of the tile kernel resident per CU, CUs of the device}
+ * (the last two are 0 for the plain kernels). */
+fvm_status fvm_batch_get_config(fvm_batch *b, int32_t config[6]);
+fvm_status fvm_get_config(fvm_solver *s, int32_t config[6]);
 void fvm_batch_destroy(fvm_batch *b);
 
 #ifdef __cplusplus

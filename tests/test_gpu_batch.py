@@ -109,3 +109,9 @@ def test_kernel_times_and_config(meshes):
     W = s.state()
     s.kernel_times(reps=3)  # timing launches leave the state unchanged
     assert np.array_equal(s.state(), W)
+    # library-chosen tile size: the tiles fill whole waves of the CUs' slots
+    for prec in ("fp32", "fp64"):
+        c = Solver(meshes[1], precision=prec, **FLOWS[1]).config()
+        slots = c["blocks_per_cu"] * c["cus"]
+        assert 64 <= c["tile_cells"] <= 160 and c["threads"] == 256 and slots > 0
+        assert c["tiles"] == -(-meshes[1].ncell // c["tile_cells"]) and c["tiles"] <= slots
