@@ -37,10 +37,11 @@ inline size_t update_tile_lds(int max_local) {
 }
 template <typename R>
 inline size_t stage_tile_lds(int max_ring_local, int max_w_local, int max_local, int max_edges) {
-  // W[MW] | union { R1[MR] | R[MW] ; grad + bounds + F + SR }
+  // W[MW] | GC[MW rounded up to even] | union { R1[MR] | R[MW] ; grad + bounds + F + SR }
   const size_t res = (size_t)(max_ring_local + max_w_local) * sizeof(SVec4<R>);
   const size_t flux = flux_tile_lds<R>(max_local, max_edges) - (size_t)max_local * sizeof(SVec4<R>);
-  return (size_t)max_w_local * sizeof(SVec4<R>) + (res > flux ? res : flux);
+  return (size_t)max_w_local * sizeof(SVec4<R>) + (size_t)((max_w_local + 1) & ~1) * sizeof(int2) +
+         (res > flux ? res : flux);
 }
 
 }  // namespace fvm

@@ -177,18 +177,20 @@ __device__ __forceinline__ void neumaier(double& s, double& c, double v) {
 
 // Green-Gauss gradient and neighbourhood bounds of one cell into the LDS
 // arrays sG[8][ML], sB[8][ML] (kernel_Gradient, gpu_solver.cu:225-383).
-// nbk: neighbour codes (>= 0 interior, -1 far field, -2 wall); fetch(k)
-// returns the state of interior neighbour k.
-template <typename R, typename Fetch>
+// nbk: neighbour codes (>= 0 interior, -1 far field, -2 wall); nbv: the
+// states of the interior neighbours (entries of boundary sides are unused).
+// Callers load all three sides at once, a boundary side reading the cell
+// itself, so the loads overlap instead of waiting on each other.
+template <typename R>
 __device__ __forceinline__ void gradient_cell(int j, const Vec4<R>& Wi, const CellGeom<R>& cg,
-                                              const int nbk[3], Fetch fetch, const FlowT<R>& f,
-                                              R* sG, R* sB, int ML) {
+                                              const int nbk[3], const Vec4<R> nbv[3],
+                                              const FlowT<R>& f, R* sG, R* sB, int ML) {
   R gx[4] = {R(0.0), R(0.0), R(0.0), R(0.0)}, gy[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
   R mx[4], mn[4];
   const R ui[4] = {Wi.d, Wi.du, Wi.dv, Wi.de};
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    const Vec4<R> c = nbk[k] >= 0 ? fetch(k) : boundary_ghost<R>(-nbk[k], Wi, cg.vx[k], cg.vy[k], f);
+    const Vec4<R> c = nbk[k] >= 0 ? nbv[k] : boundary_ghost<R>(-nbk[k], Wi, cg.vx[k], cg.vy[k], f);
     const R cc[4] = {c.d, c.du, c.dv, c.de};
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -359,8 +361,10 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
     if (a.order >= 2) {
       const int4 nb = a.nbr[d.geo0 + g];
       const int nbk[3] = {nb.x, nb.y, nb.z};
-      gradient_cell<R>(j, Wi, a.cg[d.geo0 + g], nbk, [&](int k) { return ld(W[nbk[k]]); }, f, sG,
-                       sB, ML);
+      Vec4<R> nbv[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) nbv[k] = ld(W[nbk[k] >= 0 ? nbk[k] : g]);
+      gradient_cell<R>(j, Wi, a.cg[d.geo0 + g], nbk, nbv, f, sG, sB, ML);
     }
   }
   __syncthreads();
@@ -408,15 +412,17 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
       const int g = j < nt ? d.cell0 + j : a.halo[d.halo0 + j - nt];
       const int4 nb = a.nbr[d.geo0 + g];
       const int nbk[3] = {nb.x, nb.y, nb.z};
+      const SVec4<R> ri = Rs[g];
+      SVec4<R> v[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) v[k] = Rs[nbk[k] >= 0 ? nbk[k] : g];
       R s0 = R(0.0), s1 = R(0.0), s2 = R(0.0), s3 = R(0.0);
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         if (nbk[k] >= 0) {
-          const SVec4<R> v = Rs[nbk[k]];
-          s0 += v.d; s1 += v.du; s2 += v.dv; s3 += v.de;
+          s0 += v[k].d; s1 += v[k].du; s2 += v[k].dv; s3 += v[k].de;
         }
       }
-      const SVec4<R> ri = Rs[g];
       const R den = R(1.0) + eps * R(nb.w);
       sR1[j] = SVec4<R>{(s0 * eps + ri.d) / den, (s1 * eps + ri.du) / den,
                         (s2 * eps + ri.dv) / den, (s3 * eps + ri.de) / den};
@@ -430,12 +436,14 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
     if (a.smoothing) {
       const int4 nl4 = a.nloc[d.geo0 + g];
       const int lk[3] = {nl4.x, nl4.y, nl4.z};
+      SVec4<R> v[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) v[k] = sR1[lk[k] >= 0 ? lk[k] : i];
       R s0 = R(0.0), s1 = R(0.0), s2 = R(0.0), s3 = R(0.0);
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         if (lk[k] >= 0) {
-          const SVec4<R> v = sR1[lk[k]];
-          s0 += v.d; s1 += v.du; s2 += v.dv; s3 += v.de;
+          s0 += v[k].d; s1 += v[k].du; s2 += v[k].dv; s3 += v[k].de;
         }
       }
       const R den = R(1.0) + eps * R(a.nbr[d.geo0 + g].w);
@@ -469,10 +477,14 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
 
 // --------------------------------------------------------------------------
 // One kernel per RK stage (see the header comment).
-// LDS: W[MW] | union { R1[MR] | R[MW] ;  grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME] }
-// The residual buffers of steps 1-2 are dead once W is complete, so the
-// gradient and flux buffers of steps 3-5 reuse their space (a smaller
-// footprint lets more workgroups share a CU).
+// LDS: W[MW] | GC[MW2] | union { R1[MR] | R[MW] ;  grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME] }
+// GC holds each entry's cell id and CLog, read in step 1 and reused by
+// steps 2-3 instead of the ring table.  The residual buffers of steps 1-2 are
+// dead once W is complete, so the gradient and flux buffers of steps 3-5
+// reuse their space (a smaller footprint lets more workgroups share a CU).
+// Each item issues its independent global loads together (neighbours of a
+// boundary side read the cell itself and are not added), so one item waits
+// for at most three dependent memory round trips.
 // --------------------------------------------------------------------------
 template <typename R>
 __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
@@ -486,10 +498,12 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   const FlowT<R> f = a.flow[d.icase];
   const int nt = d.ncell, NG = nt + d.nhalo, NW = NG + d.n2, NR = NW + d.n3;
   const int MR = a.max_ring_local, MW = a.max_w_local, ML = a.max_local, ME = a.max_edges;
+  const int MW2 = (MW + 1) & ~1;
   SVec4<R>* sW = reinterpret_cast<SVec4<R>*>(smem);
-  SVec4<R>* sR1 = sW + MW;
+  int2* sGC = reinterpret_cast<int2*>(sW + MW);
+  SVec4<R>* sR1 = reinterpret_cast<SVec4<R>*>(sGC + MW2);
   SVec4<R>* sR = sR1 + MR;
-  R* sG = reinterpret_cast<R*>(sW + MW);
+  R* sG = reinterpret_cast<R*>(sR1);
   R* sB = sG + 8 * ML;
   SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
   R* sS = reinterpret_cast<R*>(sF + ME);
@@ -497,6 +511,7 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   const SVec4<R>* W0 = (cur ? a.w0b : a.w0a) + d.st0;
   SVec4<R>* W0n = (cur ? a.w0a : a.w0b) + d.st0;
   const int4* lnb = a.lnb + d.lnb0;
+  const CellGeom<R>* cg = a.cg + d.geo0;
   auto gid = [&](int j) { return j < nt ? d.cell0 + j : a.rings[d.ring0 + j - nt]; };
 
   const int s0 = threadIdx.x;
@@ -506,7 +521,7 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   R area0 = R(0);
   if (s0 < nt) {
     sl0 = a.slot[d.geo0 + d.cell0 + s0];
-    area0 = a.cg[d.geo0 + d.cell0 + s0].area;
+    area0 = cg[d.cell0 + s0].area;
   }
 
   double acc = 0.0;
@@ -519,40 +534,51 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     for (int j = threadIdx.x; j < n1; j += blockDim.x) {
       const int g = gid(j);
       const SVec4<R> ri = Rin[g];
-      if (j < NW) sR[j] = ri;
       if (a.smoothing) {
         const int4 nb = a.nbr[d.geo0 + g];
         const int nbk[3] = {nb.x, nb.y, nb.z};
+        SVec4<R> v[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) v[k] = Rin[nbk[k] >= 0 ? nbk[k] : g];
         R q0 = R(0.0), q1 = R(0.0), q2 = R(0.0), q3 = R(0.0);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
           if (nbk[k] >= 0) {
-            const SVec4<R> v = Rin[nbk[k]];
-            q0 += v.d; q1 += v.du; q2 += v.dv; q3 += v.de;
+            q0 += v[k].d; q1 += v[k].du; q2 += v[k].dv; q3 += v[k].de;
           }
         }
         const R den = R(1.0) + eps * R(nb.w);
         sR1[j] = SVec4<R>{(q0 * eps + ri.d) / den, (q1 * eps + ri.du) / den,
                           (q2 * eps + ri.dv) / den, (q3 * eps + ri.de) / den};
+        if (j < NW) sGC[j] = make_int2(g, nb.w);
+      } else {
+        sGC[j] = make_int2(g, 0);
       }
+      if (j < NW) sR[j] = ri;
     }
     __syncthreads();
     // 2. sweep 2, selection and the RK update on the tile and rings 1..2
     for (int j = threadIdx.x; j < NW; j += blockDim.x) {
-      const int g = gid(j);
+      const int2 gc = sGC[j];
+      const int g = gc.x;
+      const R dtv = a.dt_in[d.st0 + g];
+      const R area = cg[g].area;
+      const SVec4<R> w0 = W0[g];
       SVec4<R> ri = sR[j];
       if (a.smoothing) {
         const int4 ln = lnb[j];
         const int lk[3] = {ln.x, ln.y, ln.z};
+        SVec4<R> v[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) v[k] = sR1[lk[k] >= 0 ? lk[k] : j];
         R q0 = R(0.0), q1 = R(0.0), q2 = R(0.0), q3 = R(0.0);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
           if (lk[k] >= 0) {
-            const SVec4<R> v = sR1[lk[k]];
-            q0 += v.d; q1 += v.du; q2 += v.dv; q3 += v.de;
+            q0 += v[k].d; q1 += v[k].du; q2 += v[k].dv; q3 += v[k].de;
           }
         }
-        const R den = R(1.0) + eps * R(a.nbr[d.geo0 + g].w);
+        const R den = R(1.0) + eps * R(gc.y);
         const R p0 = (q0 * eps + ri.d) / den, p1 = (q1 * eps + ri.du) / den;
         const R p2 = (q2 * eps + ri.dv) / den, p3 = (q3 * eps + ri.de) / den;
         if (fabs(ri.d) > fabs(p0)) ri.d = p0;
@@ -560,9 +586,7 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
         if (fabs(ri.dv) > fabs(p2)) ri.dv = p2;
         if (fabs(ri.de) > fabs(p3)) ri.de = p3;
       }
-      const R rk = a.rk * a.dt_in[d.st0 + g];
-      const R area = a.cg[d.geo0 + g].area;
-      const SVec4<R> w0 = W0[g];
+      const R rk = a.rk * dtv;
       SVec4<R> wn;
       wn.d = w0.d - rk * ri.d / area;
       wn.du = w0.du - rk * ri.du / area;
@@ -576,16 +600,23 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
       }
     }
   } else {
-    for (int j = threadIdx.x; j < NW; j += blockDim.x) sW[j] = W0[gid(j)];
+    for (int j = threadIdx.x; j < NW; j += blockDim.x) {
+      const int g = gid(j);
+      sW[j] = W0[g];
+      sGC[j] = make_int2(g, 0);
+    }
   }
   __syncthreads();
   // 3. gradient and bounds of the tile and ring 1 from the updated state
   if (a.order >= 2) {
     for (int j = threadIdx.x; j < NG; j += blockDim.x) {
       const int4 ln = lnb[j];
+      const CellGeom<R> cgj = cg[sGC[j].x];
       const int nbk[3] = {ln.x, ln.y, ln.z};
-      gradient_cell<R>(j, ld(sW[j]), a.cg[d.geo0 + gid(j)], nbk,
-                       [&](int k) { return ld(sW[nbk[k]]); }, f, sG, sB, ML);
+      Vec4<R> nbv[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) nbv[k] = ld(sW[nbk[k] >= 0 ? nbk[k] : j]);
+      gradient_cell<R>(j, ld(sW[j]), cgj, nbk, nbv, f, sG, sB, ML);
     }
   }
   __syncthreads();
