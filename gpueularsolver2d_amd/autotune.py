@@ -1,13 +1,17 @@
 """Autotuner of the tiled kernels' launch shape.
 
-The two tiled kernels run one workgroup per tile of cells; the tile size sets
-the LDS footprint, the halo overhead (halo cells are recomputed by both
-neighbouring tiles) and how many workgroups fill the 256 CUs, and the
-workgroup size sets how many passes each phase needs over the tile.  tune()
-times short runs of each candidate (tile_cells, threads) on the real mesh and
-keeps the fastest per time step; results are cached in autotune_cache.json
-next to this file, keyed by device name, precision, scheme and mesh size
-bucket.
+The tile kernels run one workgroup per tile of cells.  Three things follow from the tile
+size:
+  * the LDS footprint, and with the registers, how many workgroups share a CU;
+  * the ring overhead (cells next to a tile are recomputed by each tile that needs them);
+  * how the tiles fill the device's workgroup slots.
+
+The workgroup size sets how many passes each phase needs over the tile.  With tile_cells = 0
+the library sizes the tiles so that they fill whole waves of slots.  tune() times that choice
+at several workgroup sizes, beside explicit (tile_cells, threads) candidates, with short runs
+on the real mesh, and keeps the fastest per time step.  Results are cached in
+autotune_cache.json next to this file.  The cache key is the device name, precision, scheme
+and mesh size bucket.
 """
 from __future__ import annotations
 
@@ -19,7 +23,8 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "autotune_cache.json")
 
-DEFAULT_CANDIDATES = [(t, n) for t in (96, 128, 160, 192, 256, 320, 384, 512) for n in (128, 256)]
+DEFAULT_CANDIDATES = [(0, 128), (0, 192), (0, 256)] + [
+    (t, n) for t in (96, 128, 160, 192, 256) for n in (128, 256)]
 
 
 def _key(precision, order, smoothing, ncell, device_name):
@@ -66,9 +71,10 @@ def tune(mesh, precision="fp32", order=2, smoothing=True, candidates=None, steps
             wall = (time.perf_counter() - t0) / steps * 1e3
             ms, _ = s.timing()
             kt = s.kernel_times(reps=20)
-            row = dict(tile_cells=tile, threads=threads, ms_per_step=ms / steps,
-                       wall_ms_per_step=wall, flux_tile_ms=kt[0], update_tile_ms=kt[1],
-                       tiles=s.config()["tiles"])
+            cfg = s.config()
+            row = dict(tile_cells=cfg["tile_cells"], threads=threads, requested_tile_cells=tile,
+                       ms_per_step=ms / steps, wall_ms_per_step=wall, flux_tile_ms=kt[0],
+                       update_tile_ms=kt[1], tiles=cfg["tiles"], blocks_per_cu=cfg["blocks_per_cu"])
             s.close()
         except FvmError as e:  # e.g. the tile's LDS footprint exceeds the device limit
             row = dict(tile_cells=tile, threads=threads, error=str(e))
