@@ -20,8 +20,9 @@ Measured in one run:
                    beside FP64 vs FP64 from a 1e-12-perturbed start
   accuracy_converged   FP32 vs FP64 on a converging first-order case
   batch64          64 cases on 4 uneven meshes in one batch (FP32 Mcsu/s)
-  cpu_baseline     the FP64 CPU oracle on this host, one thread, on a bounded
-                   sample of steps of the same mesh
+  batch256         256 cases on 256 meshes of 2k-150k cells (~8.8M cells, HBM-resident)
+  cpu_baseline     the FP64 CPU oracle on this host's cores (OpenMP build), on a
+                   bounded sample of steps of the same mesh; one-thread figure beside it
 """
 import argparse
 import json
@@ -47,6 +48,11 @@ def flux_bytes(w):
 
 def update_bytes(w):
     return 25 * w + 1 * w + 24
+
+
+def progress(msg):
+    """One line per finished part on stderr (the JSON result goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def sync():
@@ -82,16 +88,27 @@ def traffic_from_profiles(workload, precision, kernel):
     return d.get(f"{workload}:{precision}:{kernel}")
 
 
-def cpu_baseline(mesh, steps):
+def cpu_baseline(mesh, steps, omp_steps):
+    """The FP64 oracle (the reference's loops restated in C) on this host's
+    cores, on a bounded sample of steps of the same mesh and scheme: the
+    all-core build (OpenMP over the per-cell and per-edge loops, the same
+    bits) is the reported baseline; the one-thread build is reported beside it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_ctypes import OracleMesh, OracleSolver
-    om = OracleMesh(mesh.x, mesh.y, mesh.tri)
-    s = OracleSolver(om, 0.8, 1.25, cfl=1.5, order=2, smoothing=True)
-    s.time_steps(1)
-    sec = s.time_steps(steps)
-    return dict(value=4.0 * mesh.ncell * steps / sec / 1e6, unit="Mcsu/s", cores=1, kind="port",
-                sample=f"{steps} time steps of the same {mesh.ncell}-cell mesh, FP64 oracle "
-                       f"(gcc -O2, one thread), {sec:.1f} s")
+    from oracle_ctypes import LIB_OMP_PATH, LIB_PATH, OracleMesh, OracleSolver, lib
+    out = {}
+    for path, n in ((LIB_PATH, steps), (LIB_OMP_PATH, omp_steps)):
+        om = OracleMesh(mesh.x, mesh.y, mesh.tri, library=path)
+        s = OracleSolver(om, 0.8, 1.25, cfl=1.5, order=2, smoothing=True)
+        s.time_steps(1)
+        sec = s.time_steps(n)
+        out[path] = (4.0 * mesh.ncell * n / sec / 1e6, lib(path).orc_threads(), n, sec)
+    v1, _, n1, s1 = out[LIB_PATH]
+    vN, cores, nN, sN = out[LIB_OMP_PATH]
+    return dict(value=vN, unit="Mcsu/s", cores=cores, kind="port",
+                sample=f"{nN} time steps of the same {mesh.ncell}-cell mesh, FP64 oracle "
+                       f"(gcc -O3 -fopenmp, {cores} threads), {sN:.1f} s",
+                single_thread=dict(value=v1, cores=1,
+                                   sample=f"{n1} time steps, gcc -O2, one thread, {s1:.1f} s"))
 
 
 def limit_cycle_accuracy(mesh, flow, t0, t1, every):
@@ -191,6 +208,62 @@ def batch64(steps):
                 dcd_max=float(np.abs(cdF - cdA)[ok].max()))
 
 
+def _mesh_job(args):
+    from gpueularsolver2d_amd import naca0012_mesh
+    n, seed = args
+    return naca0012_mesh(n, seed=seed)
+
+
+def batch256(steps, workers=16):
+    """256 cases in one batch, every case on its own mesh: sizes log-uniform in
+    [2k, 150k] triangles (mesh seed 1000 + i), M uniform in [0.3, 0.85], alpha
+    uniform in [-2, 4] deg (drawn with numpy seed 256), second order with the
+    limiter and smoothing, CFL 1.5, library-chosen launch shape.  FP32
+    throughput over `steps` steps (all state resident in HBM), then the
+    worst per-case |dCl| and |dCd| against the same batch in FP64 after the
+    same steps.  Meshes are generated in a process pool."""
+    from concurrent.futures import ProcessPoolExecutor
+    from gpueularsolver2d_amd import Batch
+    rng = np.random.default_rng(256)
+    sizes = np.exp(rng.uniform(np.log(2000), np.log(150000), 256)).astype(int)
+    mach = rng.uniform(0.3, 0.85, 256)
+    alpha = rng.uniform(-2.0, 4.0, 256)
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=min(workers, os.cpu_count() or 1)) as ex:
+        meshes = list(ex.map(_mesh_job, [(int(n), 1000 + i) for i, n in enumerate(sizes)]))
+    tgen = time.perf_counter() - t0
+    flows = [dict(mach=float(mach[i]), alpha_deg=float(alpha[i]), cfl=1.5) for i in range(256)]
+    case_mesh = list(range(256))
+    cells = sum(m.ncell for m in meshes)
+    F = Batch(meshes, case_mesh, flows, precision="fp32")
+    F.run(10)
+    sync()
+    t0 = time.perf_counter()
+    _, nF, stF = F.run(10 + steps, check=False)
+    sync()
+    t = time.perf_counter() - t0
+    kt = F.kernel_times(reps=10)
+    A = Batch(meshes, case_mesh, flows, precision="fp64")
+    _, nA, stA = A.run(10 + steps, check=False)
+    done = sum((int(nF[i]) - 10) * meshes[i].ncell for i in range(256))
+    ok = (stF == 0) & (stA == 0)
+    (clF, cdF), (clA, cdA) = F.forces(), A.forces()
+    dcl, dcd = np.abs(clF - clA), np.abs(cdF - cdA)
+    worst = int(np.argmax(np.where(ok, dcl, -1.0)))
+    achieved = 402.0 * cells / (kt[0] * 1e-3) / 1e9
+    return dict(value=round(4.0 * done / t / 1e6, 1), unit="Mcsu/s", cases=256, cells=int(cells),
+                steps=steps, launch=F.config(), stage_tile_ms=kt[0],
+                roofline_frac=round(achieved / HBM_PEAK_GBS, 4), achieved_gbs=round(achieved, 1),
+                diverged={str(i): dict(fp32_status=int(stF[i]), fp32_step=int(nF[i]),
+                                       fp64_status=int(stA[i]), fp64_step=int(nA[i]))
+                          for i in range(256) if not ok[i]},
+                dcl_max=float(dcl[ok].max()), dcl_median=float(np.median(dcl[ok])),
+                dcd_max=float(dcd[ok].max()),
+                worst_case=dict(case=worst, mesh_cells=meshes[worst].ncell, **flows[worst],
+                                cl_fp64=float(clA[worst])),
+                mesh_generation_s=round(tgen, 1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,7 +279,12 @@ def main():
                     help="step limit of the converged first-order comparison (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=2000,
                     help="steps of the 64-case batch (0 skips it)")
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--batch256-steps", type=int, default=3000,
+                    help="steps of the 256-case batch (0 skips it)")
+    ap.add_argument("--cpu-steps", type=int, default=40,
+                    help="steps of the one-thread CPU sample")
+    ap.add_argument("--cpu-omp-steps", type=int, default=200,
+                    help="steps of the all-core CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-cells", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
@@ -242,6 +320,7 @@ def main():
     t32, _ = timed_run(s32, args.warmup, args.steps)
     kt = s32.kernel_times(reps=50)
     v32 = csu / t32 / 1e6
+    progress(f"fp32 {v32:.1f} Mcsu/s")
     if args.fp32_only:
         print(json.dumps({"metric": METRIC, "value": round(v32, 1), "unit": "Mcsu/s",
                           "steps": args.steps, "kernel_ms": kt}))
@@ -255,10 +334,16 @@ def main():
 
     v64 = csu / t64 / 1e6
     vpl = 4.0 * C * plain_steps / tpl / 1e6
+    progress(f"fp64 tiled {v64:.1f}, fp64 plain {vpl:.1f} Mcsu/s")
 
     acc = limit_cycle_accuracy(mesh, flow, *args.acc_window) if args.acc_window[1] > 0 else None
+    progress("limit-cycle accuracy done")
     conv = converged_accuracy(mesh, args.conv_steps) if args.conv_steps > 0 else None
+    progress("converged-case accuracy done")
     b64 = batch64(args.batch_steps) if args.batch_steps > 0 else None
+    progress("64-case batch done")
+    b256 = batch256(args.batch256_steps) if args.batch256_steps > 0 else None
+    progress("256-case batch done")
 
     w = 4  # FP32 state and stored geometry
     if kt[1] > 0:   # two kernels per stage: flux_tile dominates
@@ -275,7 +360,8 @@ def main():
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
-        cpu = cpu_baseline(mesh, args.cpu_steps)
+        cpu = cpu_baseline(mesh, args.cpu_steps, args.cpu_omp_steps)
+        progress("cpu baseline done")
 
     out = {
         "metric": METRIC, "value": round(v32, 1), "unit": "Mcsu/s", "n_gpus": 1,
@@ -294,6 +380,7 @@ def main():
         "accuracy": acc,
         "accuracy_converged": conv,
         "batch64": b64,
+        "batch256": b256,
     }
     print(json.dumps(out))
     return 0

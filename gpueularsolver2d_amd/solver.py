@@ -240,7 +240,8 @@ class Solver:
                     tiles=int(cfg[3]), blocks_per_cu=int(cfg[4]), cus=int(cfg[5]))
 
     def kernel_times(self, reps=20):
-        """Average ms per launch of (flux_tile, update_tile)."""
+        """Average ms per launch: (stage_tile, 0) with one kernel per stage,
+        (flux_tile, update_tile) with two."""
         ms = np.zeros(2)
         _check(load_library().fvm_kernel_times(self.h, reps, _dptr(ms)))
         return float(ms[0]), float(ms[1])

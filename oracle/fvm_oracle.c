@@ -27,6 +27,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* Reference constants: gpu_fvm.cu:8-10, gpu_solver.cu:603 (Harten delta),
  * gpu_solver.cu:831 (dt cap), gpu_solver.cu:34 (block size for the residual
@@ -711,6 +714,7 @@ void orc_freestream(const orc_solver *s, double out[7])
 static void boundary(orc_solver *s)
 {
     const orc_mesh *m = s->m;
+#pragma omp parallel for schedule(static)
     for (int e = 1; e <= m->nedge; e++) {
         const orc_w *wd = &s->w[m->left[e]];
         if (m->elog[e] == 1) far_boundary(s, &s->farw[m->farid[e]], wd, m->vx[e], m->vy[e]);
@@ -734,6 +738,7 @@ static const orc_w *across(const orc_solver *s, int c, int i)
 static void gradient(orc_solver *s)
 {
     const orc_mesh *m = s->m;
+#pragma omp parallel for schedule(static)
     for (int c = 1; c <= m->ncell; c++) {
         const orc_w *Wi = &s->w[c];
         double Ui[4] = {Wi->d, Wi->du, Wi->dv, Wi->de};
@@ -791,6 +796,7 @@ static void reconstruct(const orc_solver *s, int c, double midx, double midy, or
 static void flux(orc_solver *s)
 {
     const orc_mesh *m = s->m;
+#pragma omp parallel for schedule(static)
     for (int e = 1; e <= m->nedge; e++) {
         int L = m->left[e], R = m->right[e];
         orc_w LW, RW;
@@ -830,6 +836,7 @@ static void flux(orc_solver *s)
 static void gather(orc_solver *s)
 {
     const orc_mesh *m = s->m;
+#pragma omp parallel for schedule(static)
     for (int c = 1; c <= m->ncell; c++) {
         double T = 0;
         double r[4] = {0.0, 0.0, 0.0, 0.0};
@@ -860,6 +867,7 @@ static void smooth_sweeps(orc_solver *s, double *res)
 {
     const orc_mesh *m = s->m;
     double eps = s->opts.eps;
+#pragma omp parallel for schedule(static)
     for (int c = 1; c <= m->ncell; c++) {
         const int *nb = &m->neicell[3 * c];
         for (int i = 0; i < 4; i++) {
@@ -870,6 +878,7 @@ static void smooth_sweeps(orc_solver *s, double *res)
             s->r1[4 * c + i] = (data * (eps) + res[4 * c + i]) / (1.0 + (eps) * m->clog[c]);
         }
     }
+#pragma omp parallel for schedule(static)
     for (int c = 1; c <= m->ncell; c++) {
         const int *nb = &m->neicell[3 * c];
         for (int i = 0; i < 4; i++) {
@@ -887,6 +896,7 @@ static void smooth_sweeps(orc_solver *s, double *res)
 static void rk_step(orc_solver *s, double RKc)
 {
     const orc_mesh *m = s->m;
+#pragma omp parallel for schedule(static)
     for (int c = 1; c <= m->ncell; c++) {
         double RK = RKc;
         RK *= s->dt[c];
@@ -934,6 +944,7 @@ int orc_run(orc_solver *s, int maxstep, double epsl, double *resid)
     double last = 100.0;
     while (s->step < maxstep && last > epsl) {
         s->step++;
+#pragma omp parallel for schedule(static)
         for (int c = 1; c <= m->ncell; c++) s->w0[c] = s->w[c];
         double drou = 0.0;
         for (int IR = 0; IR < 4; IR++) {
@@ -1047,6 +1058,19 @@ void orc_forces(const orc_solver *s, double *cl, double *cd)
     double q = 0.5;
     if (cl) *cl = (-Fx * s->VI + Fy * s->UI) / q;
     if (cd) *cd = (Fx * s->UI + Fy * s->VI) / q;
+}
+
+/* Threads the per-cell and per-edge loops use: 1 in the plain build, the
+ * OpenMP team size in the -fopenmp build (liboracle_omp.so, CPU baseline
+ * only).  Every parallel loop writes per-cell or per-edge outputs only, and
+ * the residual sum stays sequential, so both builds give the same bits. */
+int orc_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 double orc_time_steps(orc_solver *s, int steps)

@@ -109,6 +109,8 @@ double orc_spectral_radius(const double wl[4], const double wr[4], double vx, do
 
 /* Wall-clock of `steps` oracle steps (for the CPU baseline); returns seconds. */
 double orc_time_steps(orc_solver *s, int steps);
+/* Threads of the per-cell / per-edge loops (1 unless built with -fopenmp). */
+int orc_threads(void);
 
 #ifdef __cplusplus
 }

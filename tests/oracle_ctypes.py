@@ -13,6 +13,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
+# the same oracle with its per-cell / per-edge loops on OpenMP threads (CPU
+# baseline on all cores; the same bits as liboracle.so)
+LIB_OMP_PATH = os.path.join(ORACLE_DIR, "liboracle_omp.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -28,15 +31,15 @@ class OrcOpts(C.Structure):
                 ("global_dt", C.c_int)]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(path=LIB_PATH):
+    L = _libs.get(path)
+    if L is None:
+        if not os.path.exists(path):
             subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         L.orc_mesh_build.argtypes = [C.c_int, C.c_int, _dp, _dp, _ip, C.c_int, C.POINTER(C.c_void_p)]
         L.orc_mesh_read_dat.argtypes = [C.c_char_p, C.POINTER(C.c_void_p), C.c_int]
         L.orc_mesh_free.argtypes = [C.c_void_p]
@@ -62,8 +65,10 @@ def lib():
         L.orc_spectral_radius.restype = C.c_double
         L.orc_time_steps.argtypes = [C.c_void_p, C.c_int]
         L.orc_time_steps.restype = C.c_double
-        _lib = L
-    return _lib
+        L.orc_threads.argtypes = []
+        L.orc_threads.restype = C.c_int
+        _libs[path] = L
+    return L
 
 
 def dptr(a):
@@ -81,17 +86,18 @@ def f64(a):
 class OracleMesh:
     """Mesh topology built by the oracle (1-based arrays, index 0 unused)."""
 
-    def __init__(self, x, y, tri, literal=False):
+    def __init__(self, x, y, tri, literal=False, library=LIB_PATH):
+        self.L = L = lib(library)
         x, y = f64(x), f64(y)
         tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1)
         h = C.c_void_p()
-        st = lib().orc_mesh_build(len(x), len(tri) // 3, dptr(x), dptr(y), iptr(tri),
+        st = L.orc_mesh_build(len(x), len(tri) // 3, dptr(x), dptr(y), iptr(tri),
                                   int(literal), C.byref(h))
         if st != 0:
             raise ValueError(f"oracle mesh build failed with status {st}")
         self.h = h
         cnt = np.zeros(5, np.int32)
-        lib().orc_mesh_counts(h, iptr(cnt))
+        L.orc_mesh_counts(h, iptr(cnt))
         self.nnode, self.ncell, self.nedge, self.nwall, self.nfar = (int(v) for v in cnt)
         C1, E1 = self.ncell + 1, self.nedge + 1
         self.celledge = np.zeros(3 * C1, np.int32)
@@ -101,7 +107,7 @@ class OracleMesh:
                 ("elog", "left", "right", "farid", "wallid", "node1", "node2")}
         self.area, self.cx, self.cy = (np.zeros(C1) for _ in range(3))
         dbl = {k: np.zeros(E1) for k in ("vx", "vy", "vn", "midx", "midy")}
-        lib().orc_mesh_export(h, iptr(self.celledge), iptr(self.neicell), iptr(self.clog),
+        L.orc_mesh_export(h, iptr(self.celledge), iptr(self.neicell), iptr(self.clog),
                               *(iptr(ints[k]) for k in ("elog", "left", "right", "farid", "wallid",
                                                         "node1", "node2")),
                               dptr(self.area), dptr(self.cx), dptr(self.cy),
@@ -115,7 +121,7 @@ class OracleMesh:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_mesh_free(self.h)
+            self.L.orc_mesh_free(self.h)
             self.h = None
 
 
@@ -123,60 +129,61 @@ class OracleSolver:
     def __init__(self, mesh, mach, alpha_deg, cfl=1.5, gamma=1.4, p_in=101325.0, t_in=288.15,
                  order=2, smoothing=True, eps=0.8, global_dt=False):
         self.mesh = mesh  # keeps the mesh alive
+        self.L = mesh.L  # the library that built the mesh
         self.flow = OrcFlow(mach, alpha_deg, gamma, p_in, t_in, cfl)
         self.opts = OrcOpts(order, int(bool(smoothing)), eps, int(bool(global_dt)))
-        self.h = lib().orc_create(mesh.h, C.byref(self.flow), C.byref(self.opts))
+        self.h = self.L.orc_create(mesh.h, C.byref(self.flow), C.byref(self.opts))
         if not self.h:
             raise MemoryError("oracle solver allocation failed")
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_destroy(self.h)
+            self.L.orc_destroy(self.h)
             self.h = None
 
     def run(self, maxstep, epsl=-1.0):
         hist = np.zeros(maxstep + 1)
-        n = lib().orc_run(self.h, maxstep, epsl, dptr(hist))
-        steps = lib().orc_step_count(self.h)
+        n = self.L.orc_run(self.h, maxstep, epsl, dptr(hist))
+        steps = self.L.orc_step_count(self.h)
         return hist[: steps + 1], n
 
     def state(self):
         n = self.mesh.ncell
         out = [np.zeros(n) for _ in range(4)]
-        lib().orc_get_state(self.h, *(dptr(a) for a in out))
+        self.L.orc_get_state(self.h, *(dptr(a) for a in out))
         return np.stack(out, axis=1)
 
     def set_state(self, W):
         W = f64(W)
         cols = [f64(W[:, k]) for k in range(4)]
-        lib().orc_set_state(self.h, *(dptr(a) for a in cols))
+        self.L.orc_set_state(self.h, *(dptr(a) for a in cols))
 
     def forces(self):
         cl, cd = C.c_double(), C.c_double()
-        lib().orc_forces(self.h, C.byref(cl), C.byref(cd))
+        self.L.orc_forces(self.h, C.byref(cl), C.byref(cd))
         return cl.value, cd.value
 
     def freestream(self):
         out = np.zeros(7)
-        lib().orc_freestream(self.h, dptr(out))
+        self.L.orc_freestream(self.h, dptr(out))
         return dict(zip(("UI", "VI", "AIN", "VIN", "ROUIN", "PIS", "rhoE"), out))
 
     def eval_residual(self):
         n = self.mesh.ncell
         res, sr, dt = np.zeros(4 * n), np.zeros(n), np.zeros(n)
         grad, umax, umin = np.zeros(8 * n), np.zeros(4 * n), np.zeros(4 * n)
-        lib().orc_eval_residual(self.h, dptr(res), dptr(sr), dptr(dt), dptr(grad), dptr(umax),
+        self.L.orc_eval_residual(self.h, dptr(res), dptr(sr), dptr(dt), dptr(grad), dptr(umax),
                                 dptr(umin))
         return dict(res=res.reshape(n, 4), sr=sr, dt=dt, grad=grad.reshape(n, 2, 4),
                     umax=umax.reshape(n, 4), umin=umin.reshape(n, 4))
 
     def smooth(self, res):
         r = f64(res).reshape(-1).copy()
-        lib().orc_smooth(self.h, dptr(r))
+        self.L.orc_smooth(self.h, dptr(r))
         return r.reshape(-1, 4)
 
     def time_steps(self, steps):
-        return lib().orc_time_steps(self.h, steps)
+        return self.L.orc_time_steps(self.h, steps)
 
 
 def read_dat(path):

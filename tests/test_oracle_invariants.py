@@ -199,3 +199,17 @@ def test_forces_rotate_with_alpha(rae):
     a = np.radians(10.0)
     np.testing.assert_allclose([cl1, cd1], [-cd0 * np.sin(a) + cl0 * np.cos(a),
                                             cd0 * np.cos(a) + cl0 * np.sin(a)], rtol=1e-10, atol=1e-12)
+
+
+def test_openmp_build_gives_the_same_bits():
+    # the CPU baseline's all-core build of the oracle runs the same arithmetic
+    from oracle_ctypes import LIB_OMP_PATH, lib
+    from gpueularsolver2d_amd import naca0012_mesh
+    m = naca0012_mesh(3000, seed=7)
+    a = OracleSolver(OracleMesh(m.x, m.y, m.tri), 0.8, 1.25)
+    b = OracleSolver(OracleMesh(m.x, m.y, m.tri, library=LIB_OMP_PATH), 0.8, 1.25)
+    ha, _ = a.run(30)
+    hb, _ = b.run(30)
+    assert lib(LIB_OMP_PATH).orc_threads() >= 1
+    assert np.array_equal(a.state(), b.state())
+    assert np.array_equal(ha, hb)
