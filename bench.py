@@ -214,6 +214,19 @@ def _mesh_job(args):
     return naca0012_mesh(n, seed=seed)
 
 
+def batch256_meshes(workers=16):
+    """The 256-case workload: meshes, flows (see batch256)."""
+    from concurrent.futures import ProcessPoolExecutor
+    rng = np.random.default_rng(256)
+    sizes = np.exp(rng.uniform(np.log(2000), np.log(150000), 256)).astype(int)
+    mach = rng.uniform(0.3, 0.85, 256)
+    alpha = rng.uniform(-2.0, 4.0, 256)
+    with ProcessPoolExecutor(max_workers=min(workers, os.cpu_count() or 1)) as ex:
+        meshes = list(ex.map(_mesh_job, [(int(n), 1000 + i) for i, n in enumerate(sizes)]))
+    flows = [dict(mach=float(mach[i]), alpha_deg=float(alpha[i]), cfl=1.5) for i in range(256)]
+    return meshes, flows
+
+
 def batch256(steps, workers=16):
     """256 cases in one batch, every case on its own mesh: sizes log-uniform in
     [2k, 150k] triangles (mesh seed 1000 + i), M uniform in [0.3, 0.85], alpha
@@ -222,17 +235,10 @@ def batch256(steps, workers=16):
     throughput over `steps` steps (all state resident in HBM), then the
     worst per-case |dCl| and |dCd| against the same batch in FP64 after the
     same steps.  Meshes are generated in a process pool."""
-    from concurrent.futures import ProcessPoolExecutor
     from gpueularsolver2d_amd import Batch
-    rng = np.random.default_rng(256)
-    sizes = np.exp(rng.uniform(np.log(2000), np.log(150000), 256)).astype(int)
-    mach = rng.uniform(0.3, 0.85, 256)
-    alpha = rng.uniform(-2.0, 4.0, 256)
     t0 = time.perf_counter()
-    with ProcessPoolExecutor(max_workers=min(workers, os.cpu_count() or 1)) as ex:
-        meshes = list(ex.map(_mesh_job, [(int(n), 1000 + i) for i, n in enumerate(sizes)]))
+    meshes, flows = batch256_meshes(workers)
     tgen = time.perf_counter() - t0
-    flows = [dict(mach=float(mach[i]), alpha_deg=float(alpha[i]), cfl=1.5) for i in range(256)]
     case_mesh = list(range(256))
     cells = sum(m.ncell for m in meshes)
     F = Batch(meshes, case_mesh, flows, precision="fp32")
@@ -254,6 +260,7 @@ def batch256(steps, workers=16):
     return dict(value=round(4.0 * done / t / 1e6, 1), unit="Mcsu/s", cases=256, cells=int(cells),
                 steps=steps, launch=F.config(), stage_tile_ms=kt[0],
                 roofline_frac=round(achieved / HBM_PEAK_GBS, 4), achieved_gbs=round(achieved, 1),
+                traffic=traffic_from_profiles(f"batch256-{cells}", "fp32", "stage_tile"),
                 diverged={str(i): dict(fp32_status=int(stF[i]), fp32_step=int(nF[i]),
                                        fp64_status=int(stA[i]), fp64_step=int(nA[i]))
                           for i in range(256) if not ok[i]},
@@ -290,10 +297,23 @@ def main():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--fp32-only", action="store_true",
                     help="time only the FP32 path (for profiler runs)")
+    ap.add_argument("--batch256-fp32-only", action="store_true",
+                    help="only run the FP32 256-case batch for --batch256-steps steps (for "
+                         "profiler runs)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run one device copy kernel of this many MiB and exit (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.batch256_fp32_only:
+        # the FP32 256-case batch alone (for profiler runs)
+        from gpueularsolver2d_amd import Batch
+        meshes, flows = batch256_meshes()
+        b = Batch(meshes, list(range(256)), flows, precision="fp32")
+        _, n, st = b.run(args.batch256_steps, check=False)
+        kt = b.kernel_times(reps=5)
+        print(json.dumps({"batch256_steps": args.batch256_steps, "cells": sum(m.ncell for m in meshes),
+                          "stage_tile_ms": kt[0], "launch": b.config()}))
+        return 0
     if args.copy_mb > 0:
         import torch
         x = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda")

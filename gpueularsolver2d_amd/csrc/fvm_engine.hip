@@ -363,7 +363,11 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
   if (o.kernels != FVM_KERNELS_PLAIN) {
     if (o.tile_cells > 8192)
       throw Error{FVM_ERR_ARGUMENT, "opts.tile_cells must be at most 8192"};
-    threads_ = o.threads > 0 ? o.threads : 256;
+    // default workgroup size from the launch-shape sweep on MI355X: the FP32
+    // stage kernel (70-80 VGPRs) gains from 512-thread tiles (less ring
+    // recomputation per cell, 3 workgroups per CU); the FP64 kernels (~140
+    // VGPRs, 3 waves per SIMD) keep 256
+    threads_ = o.threads > 0 ? o.threads : (fused_ && o.precision == FVM_FP32 ? 512 : 256);
     if (o.tile_cells > 0)
       tile_cells_ = o.tile_cells;
     else
@@ -412,12 +416,14 @@ size_t Batch::tile_lds() const {
 // fewer workgroups than slots are not spread evenly over the CUs, so the
 // tile size is chosen to make the batch's tiles fill whole waves of slots:
 // the smallest tile in [kMinTile, kMaxTile] whose tile count fits in the
-// number of waves that kMaxTile-cell tiles need.  The slots per CU come from
-// the occupancy of the tile kernel at kMaxTile (registers and LDS), which
-// smaller tiles can only raise.
+// number of waves that kMaxTile-cell tiles need.  Both bounds scale with the
+// workgroup size (0.25 and 0.625 cells per thread: one pass of the threads
+// covers a tile and its rings).  The slots per CU come from the occupancy of
+// the tile kernel at kMaxTile (registers and LDS), which smaller tiles can
+// only raise.
 template <typename R>
 int Batch::auto_tile_cells() {
-  constexpr int kMinTile = 64, kMaxTile = 160;
+  const int kMinTile = threads_ / 4, kMaxTile = threads_ * 5 / 8;
   const int nmesh = (int)topo_.size();
   tiled_.clear();
   for (int m = 0; m < nmesh; m++) tiled_.push_back(build_tiled(topo_[m], kMaxTile));

@@ -74,8 +74,10 @@ def test_fp64_bitwise_vs_oracle(rae3k, kernels, case):
 def test_tiled_fp64_matches_oracle_naca(naca3k):
     steps = 60
     orc, hist_o = oracle_run(naca3k, steps, 0.8, 1.25, 1.5, order=2, smoothing=True)
-    for kernels, tile in (("tiled", 32), ("tiled", 160), ("split", 48)):
-        s = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp64", kernels=kernels, tile_cells=tile)
+    for kernels, tile, threads in (("tiled", 32, 64), ("tiled", 160, 256), ("split", 48, 128),
+                                   ("tiled", 300, 512), ("tiled", 0, 0)):
+        s = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp64", kernels=kernels, tile_cells=tile,
+                   threads=threads)
         hist, n, st = s.run(steps)
         assert st == 0 and n == steps
         assert np.array_equal(s.state(), orc.state())
@@ -86,10 +88,14 @@ def test_fp32_one_and_two_kernel_paths_agree(naca3k):
     # give the same FP32 bits as the two-kernel path
     a = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp32", kernels="tiled")
     b = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp32", kernels="split")
+    c = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp32", kernels="tiled", tile_cells=48,
+               threads=128)
     ha, _, _ = a.run(50)
     hb, _, _ = b.run(50)
+    c.run(50)
     assert np.array_equal(a.state(), b.state())
-    np.testing.assert_array_equal(ha, hb)
+    assert np.array_equal(a.state(), c.state())
+    np.testing.assert_allclose(ha, hb, rtol=1e-13, atol=0)
 
 
 def test_fp32_first_order_converged_vs_oracle(naca3k):
