@@ -39,70 +39,19 @@ static const double kRK[4] = {0.0833, 0.2069, 0.4265, 1.0};  // gpu_fvm.cu:10
 // ---------------------------------------------------------------------------
 // finalize_step: per case residual and stop test
 // ---------------------------------------------------------------------------
-// Adds the compensated pair (s2, c2) to (s, c): the exact rounding error of
-// s + s2 (Knuth's two-sum) joins the compensation.
-__device__ __forceinline__ void add_pair(double& s, double& c, double s2, double c2) {
-  const double t = s + s2;
-  const double bb = t - s;
-  const double e = (s - (t - bb)) + (s2 - bb);
-  c = (c + c2) + e;
-  s = t;
-}
-
-// Closes the time step of every active case after the last stage, one
-// workgroup of kFinalizeThreads per case: the FP64 residual from the tiles'
-// partial sums of (rho - rho0)^2, Resd = sqrt((drou + 1e-32) / drou_of_step_1),
-// the history entry and the stop test step < max_steps && Resd > epsl
-// (gpu_solver.cu:2200, :2403-2433).  The sum is compensated and its order is
-// fixed: thread t adds partials t, t + 256, ... (Neumaier), then a butterfly
-// over the wave combines lane pairs lower lane first (so both lanes of a pair
-// hold bitwise the same result), then the four waves in order.
-// A separate launch: the kernel boundary orders the tiles' partial sums
-// before the close without a device-scope fence, which on MI355X would write
-// back a whole L2 per workgroup.
-// flip_w0: the step's new state went to the other step-start buffer.
+// Closes the time step of every active case (one workgroup per case, see
+// close_case): after the last stage of each step in the two-kernel and plain
+// paths, and after the last step of each batch of steps in the
+// one-kernel-per-stage path, whose other steps close inside the next step's
+// first launch.  A separate launch: the kernel boundary orders the tiles'
+// partial sums before the close without a device-scope fence, which on MI355X
+// would write back a whole L2 per workgroup.
 constexpr int kFinalizeThreads = 256;
 
 __global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(CaseCtl* ctl, const double* partial,
                                                                     double* hist, int hist_stride,
                                                                     int flip_w0) {
-  CaseCtl& c = ctl[blockIdx.x];
-  if (!c.active) return;
-  const int n = c.ntile;
-  const double* p = partial + c.tile0;
-  double s = 0.0, cc = 0.0;
-  for (int k = threadIdx.x; k < n; k += kFinalizeThreads) neumaier(s, cc, p[k]);
-  const int lane = threadIdx.x & 63;
-  for (int off = 1; off < 64; off <<= 1) {
-    const double s2 = __shfl_xor(s, off, 64), c2 = __shfl_xor(cc, off, 64);
-    if (lane & off) {  // this lane is the upper one of the pair
-      double ls = s2, lc = c2;
-      add_pair(ls, lc, s, cc);
-      s = ls;
-      cc = lc;
-    } else {
-      add_pair(s, cc, s2, c2);
-    }
-  }
-  __shared__ double ws[kFinalizeThreads / 64], wc[kFinalizeThreads / 64];
-  if (lane == 0) {
-    ws[threadIdx.x >> 6] = s;
-    wc[threadIdx.x >> 6] = cc;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  double S = ws[0], Cc = wc[0];
-  for (int k = 1; k < kFinalizeThreads / 64; k++) add_pair(S, Cc, ws[k], wc[k]);
-  const double drou = S + Cc;
-  const int step = c.step + 1;
-  c.step = step;
-  if (flip_w0) c.w0_cur ^= 1;
-  if (step == 1) c.epsm = drou;
-  const double resd = sqrt((drou + 1.0E-32) / c.epsm);
-  c.resd = resd;
-  if (step < hist_stride) hist[(size_t)blockIdx.x * hist_stride + step] = resd;
-  if (!isfinite(resd)) c.status = FVM_ERR_NAN;
-  if (!(step < c.max_steps && resd > c.epsl)) c.active = 0;
+  close_case(ctl[blockIdx.x], partial, hist, hist_stride, blockIdx.x, flip_w0);
 }
 
 // ---------------------------------------------------------------------------
@@ -385,6 +334,9 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
   else
     setup_tiled<float>();
   steps_per_graph_ = o.steps_per_graph > 0 ? o.steps_per_graph : 20;
+  // the one-kernel-per-stage graph alternates the step-start buffers, so it
+  // holds an even number of steps
+  if (fused_) steps_per_graph_ += steps_per_graph_ & 1;
   HIP_OK(hipStreamSynchronize(stream_));
 }
 
@@ -847,14 +799,24 @@ TiledArgs<R> Batch::tiled_args() {
   a.lnb = d_lnb_;
   a.max_ring_local = max_ring_local_;
   a.max_w_local = max_w_local_;
+  a.hist = d_hist_;
+  a.hist_stride = hist_stride_;
   return a;
 }
 
 // One kernel per stage.  Residual and time-step buffers alternate between
 // (a) and (b) across the stages; a step is: stages 1..3 (each applies the
 // previous stage's update), then the launch that applies the last update,
-// writes the new step-start state and computes stage 0 of the next step, then
-// the step close.  Stage 0 of the very first step comes from fused_init.
+// writes the new step-start state and computes stage 0 of the next step.
+// Stage 0 of the very first step comes from fused_init.  Each step's close
+// (residual, history, stop test) runs at the end of the next step's first
+// launch, in the workgroup of each case's first tile, and the last step's in
+// finalize_kernel.  That first launch neither reads nor writes what the close
+// updates except `active`: a case that stops at the close of step n may still
+// run it for step n + 1, which only writes residual buffers, and the launches
+// after it see the case stopped.  So the step-start state of step s of this
+// batch is in buffer w0_cur ^ (s & 1) (w0_cur is not changed on the device),
+// and nsteps follows an even number of steps of the same run.
 template <typename R>
 void Batch::enqueue_fused_steps(int nsteps) {
   TiledArgs<R> a = tiled_args<R>();
@@ -864,6 +826,7 @@ void Batch::enqueue_fused_steps(int nsteps) {
   R* Db = static_cast<R*>(d_dt_b_);
   a.apply_update = 1;
   for (int s = 0; s < nsteps; s++) {
+    a.parity = s & 1;
     for (int k = 1; k <= 4; k++) {
       const bool from_a = (k & 1) == 1;  // stages 1 and 3 read (a)
       a.r_in = from_a ? Ra : Rb;
@@ -872,10 +835,13 @@ void Batch::enqueue_fused_steps(int nsteps) {
       a.dt = from_a ? Db : Da;
       a.rk = (R)kRK[k - 1];
       a.close_step = k == 4;
+      a.close_prev = k == 1 && s > 0;
       HIP_OK(launch_stage_tile<R>(a, threads_, lds_s_, stream_));
     }
+  }
+  if (nsteps > 0) {
     hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(kFinalizeThreads), 0, stream_, d_ctl_,
-                       d_partial_, d_hist_, hist_stride_, 1);
+                       d_partial_, d_hist_, hist_stride_, 0);
     HIP_OK(hipGetLastError());
   }
 }
@@ -1081,6 +1047,11 @@ void Batch::run(int max_steps, double epsl, double* hist, int* steps_done, int* 
   for (int c = 0; c < ncase; c++) {
     if (steps_done) steps_done[c] = h_ctl_[c].step;
     if (case_status) case_status[c] = h_ctl_[c].status;
+  }
+  if (fused_) {
+    // each step of this run alternated the case's step-start buffer
+    for (int c = 0; c < ncase; c++) h_ctl_[c].w0_cur ^= (h_ctl_[c].step - before[c]) & 1;
+    HIP_OK(hipMemcpy(d_ctl_, h_ctl_.data(), ncase * sizeof(CaseCtl), hipMemcpyHostToDevice));
   }
 }
 

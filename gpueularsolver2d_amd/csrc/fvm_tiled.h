@@ -142,6 +142,10 @@ struct TiledArgs {
   int apply_update;           // 0 only for the first launch of a run
   int close_step;             // 1: the update closes a time step (new W0 + partial)
   int max_ring_local, max_w_local;
+  int parity;                 // step parity within the run: step-start buffer = w0_cur ^ parity
+  int close_prev;             // 1: the workgroup of each case's first tile closes the previous step
+  double* hist;               // residual history, hist_stride per case
+  int hist_stride;
 };
 
 __device__ __forceinline__ int tile_of_block(int bid, int ntiles) {
@@ -169,6 +173,65 @@ __device__ __forceinline__ void neumaier(double& s, double& c, double v) {
   else
     c += (v - t) + s;
   s = t;
+}
+
+// Adds the compensated pair (s2, c2) to (s, c): the exact rounding error of
+// s + s2 (Knuth's two-sum) joins the compensation.
+__device__ __forceinline__ void add_pair(double& s, double& c, double s2, double c2) {
+  const double t = s + s2;
+  const double bb = t - s;
+  const double e = (s - (t - bb)) + (s2 - bb);
+  c = (c + c2) + e;
+  s = t;
+}
+
+// Closes one time step of case c from its tiles' partial sums of
+// (rho - rho0)^2 (all threads of one workgroup; blockDim a multiple of 64):
+// Resd = sqrt((drou + 1e-32) / drou_of_step_1), the history entry and the
+// stop test step < max_steps && Resd > epsl (gpu_solver.cu:2200,
+// :2403-2433).  The sum is compensated and its order fixed: thread t adds
+// partials t, t + blockDim, ... (Neumaier), a butterfly over each wave
+// combines lane pairs lower lane first (both lanes of a pair then hold bitwise
+// the same value), then thread 0 adds the waves in order.
+// flip_w0: the step's new state went to the other step-start buffer.
+__device__ inline void close_case(CaseCtl& c, const double* partial, double* hist, int hist_stride,
+                                  int icase, int flip_w0) {
+  if (!c.active) return;
+  const int n = c.ntile;
+  const double* p = partial + c.tile0;
+  double s = 0.0, cc = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) neumaier(s, cc, p[k]);
+  const int lane = threadIdx.x & 63;
+  for (int off = 1; off < 64; off <<= 1) {
+    const double s2 = __shfl_xor(s, off, 64), c2 = __shfl_xor(cc, off, 64);
+    if (lane & off) {  // this lane is the upper one of the pair
+      double ls = s2, lc = c2;
+      add_pair(ls, lc, s, cc);
+      s = ls;
+      cc = lc;
+    } else {
+      add_pair(s, cc, s2, c2);
+    }
+  }
+  __shared__ double ws[16], wc[16];
+  if (lane == 0) {
+    ws[threadIdx.x >> 6] = s;
+    wc[threadIdx.x >> 6] = cc;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double S = ws[0], C = wc[0];
+  for (int k = 1; k < (int)(blockDim.x >> 6); k++) add_pair(S, C, ws[k], wc[k]);
+  const double drou = S + C;
+  const int step = c.step + 1;
+  c.step = step;
+  if (flip_w0) c.w0_cur ^= 1;
+  if (step == 1) c.epsm = drou;
+  const double resd = sqrt((drou + 1.0E-32) / c.epsm);
+  c.resd = resd;
+  if (step < hist_stride) hist[(size_t)icase * hist_stride + step] = resd;
+  if (!isfinite(resd)) c.status = FVM_ERR_NAN;
+  if (!(step < c.max_steps && resd > c.epsl)) c.active = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -507,7 +570,7 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   R* sB = sG + 8 * ML;
   SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
   R* sS = reinterpret_cast<R*>(sF + ME);
-  const int cur = cc.w0_cur;
+  const int cur = cc.w0_cur ^ a.parity;
   const SVec4<R>* W0 = (cur ? a.w0b : a.w0a) + d.st0;
   SVec4<R>* W0n = (cur ? a.w0a : a.w0b) + d.st0;
   const int4* lnb = a.lnb + d.lnb0;
@@ -628,6 +691,15 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   if (a.close_step && a.apply_update) {
     const double s = block_sum(acc);
     if (threadIdx.x == 0) a.partial[t] = s;
+  }
+  // the previous step's close, by the workgroup of the case's first tile once
+  // its own tile is done.  The other tiles of this launch read `active` only
+  // (a tile that sees the old value computes a residual that the next launch,
+  // seeing the case stopped, ignores) and take the step-start buffer from the
+  // launch's parity, not from the control block.
+  if (a.close_prev && t == cc.tile0) {
+    __syncthreads();
+    close_case(a.ctl[d.icase], a.partial, a.hist, a.hist_stride, d.icase, 0);
   }
 }
 
