@@ -15,6 +15,8 @@ Measured in one run:
   roofline         dominant kernel (stage_tile, one launch per RK stage):
                    algorithmic bytes per launch / average launch time from HIP
                    events on the solver stream
+  config_c1 / c2   the small configurations: 3k cells first order against the
+                   oracle (bitwise), 50k cells 5000 steps in FP64 and FP32
   accuracy         FP32 vs FP64 (the FP64 path equals the FP64 oracle bit for
                    bit) on the workload's limit cycle: window means of Cl/Cd,
                    beside FP64 vs FP64 from a 1e-12-perturbed start
@@ -208,6 +210,58 @@ def batch64(steps):
                 dcd_max=float(np.abs(cdF - cdA)[ok].max()))
 
 
+def config_c1(steps=2000):
+    """Config C1: the ~3k-triangle NACA 0012 mesh (seed 1), M 0.5, alpha 0,
+    first order, no smoothing, CFL 1.5, 2000 steps.  The GPU FP64 run against
+    the FP64 oracle on this host's cores (the same arithmetic): whether every
+    cell's state is bitwise equal, the largest history difference, Cl, Cd."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_ctypes import LIB_OMP_PATH, OracleMesh, OracleSolver
+    from gpueularsolver2d_amd import Solver, naca0012_mesh
+    m = naca0012_mesh(3000, seed=1)
+    kw = dict(mach=0.5, alpha_deg=0.0, cfl=1.5, order=1, smoothing=False)
+    s = Solver(m, precision="fp64", **kw)
+    h, n, _ = s.run(steps)
+    o = OracleSolver(OracleMesh(m.x, m.y, m.tri, library=LIB_OMP_PATH), 0.5, 0.0, cfl=1.5, order=1,
+                     smoothing=False)
+    ho, _ = o.run(steps)
+    cl, cd = s.forces()
+    clo, cdo = o.forces()
+    return dict(cells=m.ncell, steps=int(n), resd=float(h[-1]), cl=cl, cd=cd,
+                state_bitwise_equal_to_oracle=bool(np.array_equal(s.state(), o.state())),
+                hist_max_rel_diff=float(np.max(np.abs(h[1:] - ho[1:]) / np.abs(ho[1:]))),
+                dcl_vs_oracle=cl - clo, dcd_vs_oracle=cd - cdo)
+
+
+def config_c2(steps=5000):
+    """Config C2: the ~50k-triangle NACA 0012 mesh (seed 2), M 0.8,
+    alpha 1.25, second order with the limiter and smoothing, CFL 1.5, 5000
+    steps, in FP64 (equal to the oracle) and in FP32: throughput of both and
+    the FP32 errors after the 5000 steps (density rel. L2, Cl, Cd).  The flow
+    is still in its transient at step 5000 (see the limit-cycle comparison for
+    what instantaneous differences mean on such flows)."""
+    from gpueularsolver2d_amd import Solver, naca0012_mesh
+    m = naca0012_mesh(50000, seed=2)
+    flow = dict(mach=0.8, alpha_deg=1.25, cfl=1.5)
+    out = dict(cells=m.ncell, steps=steps)
+    sol = {}
+    for prec in ("fp64", "fp32"):
+        s = Solver(m, precision=prec, **flow)
+        sync()
+        t0 = time.perf_counter()
+        h, n, _ = s.run(steps)
+        sync()
+        t = time.perf_counter() - t0
+        out[prec] = dict(value=round(4.0 * m.ncell * steps / t / 1e6, 1), unit="Mcsu/s",
+                         resd=float(h[-1]), launch=s.config())
+        sol[prec] = s
+    (cl, cd), (clF, cdF) = sol["fp64"].forces(), sol["fp32"].forces()
+    WA, WF = sol["fp64"].state(), sol["fp32"].state()
+    out.update(cl_fp64=cl, cd_fp64=cd, dcl=clF - cl, dcd=cdF - cd,
+               rho_rel_l2=float(np.linalg.norm(WF[:, 0] - WA[:, 0]) / np.linalg.norm(WA[:, 0])))
+    return out
+
+
 def _mesh_job(args):
     from gpueularsolver2d_amd import naca0012_mesh
     n, seed = args
@@ -284,6 +338,8 @@ def main():
                          "(end 0 skips it)")
     ap.add_argument("--conv-steps", type=int, default=40000,
                     help="step limit of the converged first-order comparison (0 skips it)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the C1 (3k, first order) and C2 (50k, 5000 steps) lines")
     ap.add_argument("--batch-steps", type=int, default=2000,
                     help="steps of the 64-case batch (0 skips it)")
     ap.add_argument("--batch256-steps", type=int, default=3000,
@@ -356,6 +412,9 @@ def main():
     vpl = 4.0 * C * plain_steps / tpl / 1e6
     progress(f"fp64 tiled {v64:.1f}, fp64 plain {vpl:.1f} Mcsu/s")
 
+    c1 = config_c1() if not args.no_configs else None
+    c2 = config_c2() if not args.no_configs else None
+    progress("configs C1, C2 done")
     acc = limit_cycle_accuracy(mesh, flow, *args.acc_window) if args.acc_window[1] > 0 else None
     progress("limit-cycle accuracy done")
     conv = converged_accuracy(mesh, args.conv_steps) if args.conv_steps > 0 else None
@@ -397,6 +456,8 @@ def main():
         "fp64_tiled": {"value": round(v64, 1), "kernel_ms": kt64[0]},
         "fp64_plain": {"value": round(vpl, 1), "steps": plain_steps},
         "fp32_over_fp64_plain": round(v32 / vpl, 2),
+        "config_c1": c1,
+        "config_c2": c2,
         "accuracy": acc,
         "accuracy_converged": conv,
         "batch64": b64,

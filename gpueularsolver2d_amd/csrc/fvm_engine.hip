@@ -181,6 +181,7 @@ class Batch {
   void setup_tiled();
   template <typename R>
   size_t tile_lds() const;
+  int max_local() const;
   template <typename R>
   int auto_tile_cells();
   void setup_plain();
@@ -359,8 +360,16 @@ size_t Batch::tile_lds() const {
     MR = std::max(MR, t.max_ring_local);
     MW = std::max(MW, t.max_w_local);
   }
-  return fused_ ? stage_tile_lds<R>(MR, MW, ML, ME)
+  const int SL = stage_slope_stride(ML);
+  return fused_ ? stage_tile_lds<R>(MR, MW, SL > 0 ? SL : ML, ME)
                 : std::max(flux_tile_lds<R>(ML, ME), update_tile_lds<R>(ML));
+}
+
+// The largest tile + ring-1 count over the batch's meshes.
+int Batch::max_local() const {
+  int ML = 0;
+  for (auto& t : tiled_) ML = std::max(ML, t.max_local);
+  return ML;
 }
 
 // Tile size when the caller leaves it to the library.  Workgroups that
@@ -383,7 +392,7 @@ int Batch::auto_tile_cells() {
   int per_cu = 0;
   if (lds <= 160 * 1024) {
     HIP_OK(set_tile_lds_limit<R>(lds));
-    HIP_OK(tile_blocks_per_cu<R>(fused_ ? kStageTile : kFluxTile, threads_, lds, &per_cu));
+    HIP_OK(tile_blocks_per_cu<R>(fused_ ? kStageTile : kFluxTile, threads_, lds, max_local(), &per_cu));
   }
   if (per_cu < 1) return kMaxTile;
   auto ntiles = [&](int t) {
@@ -418,13 +427,14 @@ void Batch::setup_tiled() {
   }
   lds_a_ = flux_tile_lds<R>(ML, ME);
   lds_b_ = update_tile_lds<R>(ML);
-  lds_s_ = stage_tile_lds<R>(max_ring_local_, max_w_local_, ML, ME);
+  const int SL = stage_slope_stride(ML);
+  lds_s_ = stage_tile_lds<R>(max_ring_local_, max_w_local_, SL > 0 ? SL : ML, ME);
   const size_t lds_max = std::max(lds_a_, fused_ ? lds_s_ : lds_b_);
   if (lds_max > 160 * 1024)
     throw Error{FVM_ERR_ARGUMENT, "tile_cells too large: the tile kernels need " +
                                       std::to_string(lds_max) + " bytes of LDS"};
   HIP_OK(set_tile_lds_limit<R>(std::max(lds_max, lds_b_)));
-  HIP_OK(tile_blocks_per_cu<R>(fused_ ? kStageTile : kFluxTile, threads_, fused_ ? lds_s_ : lds_a_,
+  HIP_OK(tile_blocks_per_cu<R>(fused_ ? kStageTile : kFluxTile, threads_, fused_ ? lds_s_ : lds_a_, ML,
                                &blocks_per_cu_));
   // geometry offsets per mesh
   std::vector<int> geo0(nmesh), halo0(nmesh), edge0(nmesh), wall0(nmesh), ring0(nmesh), lnb0(nmesh);

@@ -23,8 +23,9 @@ hipError_t set_tile_lds_limit(size_t bytes);
 // Workgroups of one tile kernel that a CU holds at once with this workgroup
 // size and dynamic LDS (registers and LDS both counted).
 enum TileKernel { kFluxTile = 0, kUpdateTile = 1, kStageTile = 2 };
+// max_local selects the stage kernel's instantiation (see stage_slope_stride).
 template <typename R>
-hipError_t tile_blocks_per_cu(TileKernel k, int threads, size_t lds, int* blocks);
+hipError_t tile_blocks_per_cu(TileKernel k, int threads, size_t lds, int max_local, int* blocks);
 
 template <typename R>
 inline size_t flux_tile_lds(int max_local, int max_edges) {
@@ -35,6 +36,17 @@ template <typename R>
 inline size_t update_tile_lds(int max_local) {
   return (size_t)max_local * sizeof(SVec4<R>);
 }
+// Slope row stride of the stage kernel instantiation used for tiles whose
+// largest local count (tile + ring 1) is max_local: the smallest compiled
+// stride that holds it, or 0 (the runtime stride max_local).
+inline int stage_slope_stride(int max_local) {
+  for (int s : {192, 320, 448})
+    if (max_local <= s) return s;
+  return 0;
+}
+
+// max_local: the slope row stride (stage_slope_stride's value, or max_local
+// when that is 0).
 template <typename R>
 inline size_t stage_tile_lds(int max_ring_local, int max_w_local, int max_local, int max_edges) {
   // W[MW] | GC[MW rounded up to even] | union { R1[MR] | R[MW] ; grad + bounds + F + SR }

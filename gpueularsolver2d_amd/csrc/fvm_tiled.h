@@ -541,6 +541,7 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
 // --------------------------------------------------------------------------
 // One kernel per RK stage (see the header comment).
 // LDS: W[MW] | GC[MW2] | union { R1[MR] | R[MW] ;  grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME] }
+// (ML: the slope row stride, S or max_local)
 // GC holds each entry's cell id and CLog, read in step 1 and reused by
 // steps 2-3 instead of the ring table.  The residual buffers of steps 1-2 are
 // dead once W is complete, so the gradient and flux buffers of steps 3-5
@@ -549,7 +550,10 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
 // boundary side read the cell itself and are not added), so one item waits
 // for at most three dependent memory round trips.
 // --------------------------------------------------------------------------
-template <typename R>
+// S > 0: the slope rows are S apart (S >= max_local), a compile-time stride,
+// so the 16 LDS accesses per cell share one address register with immediate
+// offsets; S == 0: the stride is max_local.
+template <typename R, int S>
 __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int t = tile_of_block(blockIdx.x, a.ntiles);
@@ -560,7 +564,8 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   if (a.apply_update ? !cc.active : cc.status != FVM_OK) return;
   const FlowT<R> f = a.flow[d.icase];
   const int nt = d.ncell, NG = nt + d.nhalo, NW = NG + d.n2, NR = NW + d.n3;
-  const int MR = a.max_ring_local, MW = a.max_w_local, ML = a.max_local, ME = a.max_edges;
+  const int MR = a.max_ring_local, MW = a.max_w_local, ME = a.max_edges;
+  const int ML = S > 0 ? S : a.max_local;
   const int MW2 = (MW + 1) & ~1;
   SVec4<R>* sW = reinterpret_cast<SVec4<R>*>(smem);
   int2* sGC = reinterpret_cast<int2*>(sW + MW);

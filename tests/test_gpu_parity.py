@@ -74,8 +74,9 @@ def test_fp64_bitwise_vs_oracle(rae3k, kernels, case):
 def test_tiled_fp64_matches_oracle_naca(naca3k):
     steps = 60
     orc, hist_o = oracle_run(naca3k, steps, 0.8, 1.25, 1.5, order=2, smoothing=True)
+    # slope row strides 192, 320 and 448 and the runtime stride (tile + ring 1 > 448)
     for kernels, tile, threads in (("tiled", 32, 64), ("tiled", 160, 256), ("split", 48, 128),
-                                   ("tiled", 300, 512), ("tiled", 0, 0)):
+                                   ("tiled", 300, 512), ("tiled", 400, 512), ("tiled", 0, 0)):
         s = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp64", kernels=kernels, tile_cells=tile,
                    threads=threads)
         hist, n, st = s.run(steps)
