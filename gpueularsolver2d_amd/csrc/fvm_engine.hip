@@ -380,37 +380,51 @@ int Batch::max_local() const {
 // number of waves that kMaxTile-cell tiles need.  Both bounds scale with the
 // workgroup size (0.25 and 0.625 cells per thread: one pass of the threads
 // covers a tile and its rings).  The slots per CU come from the occupancy of
-// the tile kernel at kMaxTile (registers and LDS), which smaller tiles can
-// only raise.
+// the tile kernel (registers and LDS) at kMaxTile; smaller tiles need less
+// LDS and may fit more workgroups, so the choice is checked once at the
+// chosen size and redone with the larger slot count if it grew.
 template <typename R>
 int Batch::auto_tile_cells() {
   const int kMinTile = threads_ / 4, kMaxTile = threads_ * 5 / 8;
   const int nmesh = (int)topo_.size();
-  tiled_.clear();
-  for (int m = 0; m < nmesh; m++) tiled_.push_back(build_tiled(topo_[m], kMaxTile));
-  const size_t lds = tile_lds<R>();
-  int per_cu = 0;
-  if (lds <= 160 * 1024) {
-    HIP_OK(set_tile_lds_limit<R>(lds));
-    HIP_OK(tile_blocks_per_cu<R>(fused_ ? kStageTile : kFluxTile, threads_, lds, max_local(), &per_cu));
-  }
-  if (per_cu < 1) return kMaxTile;
+  auto slots_at = [&](int t) {
+    tiled_.clear();
+    for (int m = 0; m < nmesh; m++) tiled_.push_back(build_tiled(topo_[m], t));
+    const size_t lds = tile_lds<R>();
+    int per_cu = 0;
+    if (lds <= 160 * 1024) {
+      HIP_OK(set_tile_lds_limit<R>(lds));
+      HIP_OK(tile_blocks_per_cu<R>(fused_ ? kStageTile : kFluxTile, threads_, lds, max_local(),
+                                   &per_cu));
+    }
+    return (long)per_cu * cus_;
+  };
   auto ntiles = [&](int t) {
     long n = 0;
     for (auto& c : cases_) n += (topo_[c.mesh].ncell + t - 1) / t;
     return n;
   };
-  const long slots = (long)per_cu * cus_;
+  // the smallest tile whose count fits in `waves` waves of `slots` slots
+  auto fill = [&](long slots, long waves) {
+    int lo = kMinTile, hi = kMaxTile;
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (ntiles(mid) <= waves * slots)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    return lo;
+  };
+  const long slots = slots_at(kMaxTile);
+  if (slots < 1) return kMaxTile;
   const long waves = (ntiles(kMaxTile) + slots - 1) / slots;
-  int lo = kMinTile, hi = kMaxTile;
-  while (lo < hi) {
-    const int mid = (lo + hi) / 2;
-    if (ntiles(mid) <= waves * slots)
-      hi = mid;
-    else
-      lo = mid + 1;
-  }
-  return lo;
+  const int t = fill(slots, waves);
+  const long slots_t = slots_at(t);
+  if (slots_t <= slots) return t;
+  // a smaller tile keeps at least slots_t slots; a larger one might not
+  const int t2 = fill(slots_t, (ntiles(kMaxTile) + slots_t - 1) / slots_t);
+  return t2 < t ? t2 : t;
 }
 
 template <typename R>
