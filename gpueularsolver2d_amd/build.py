@@ -4,8 +4,8 @@
 
 FP64 translation units are compiled with -ffp-contract=off so every multiply
 and add is rounded separately, matching the FP64 oracle bit for bit; the FP32
-tiled kernels use fused multiply-adds, reciprocal-multiply division and the
-hardware FP32 reciprocal and square root.  Packed FP32 instructions are
+tiled kernels use fused multiply-adds within each expression,
+reciprocal-multiply division and the hardware FP32 reciprocal and square root.  Packed FP32 instructions are
 disabled for all device code.
 """
 import os
@@ -30,12 +30,15 @@ SOURCES = [
     ("fvm_mesh.cpp", ["-ffp-contract=off", "-x", "hip"]),
     ("fvm_plain.hip", ["-ffp-contract=off"]),
     ("fvm_tiled64.hip", ["-ffp-contract=off"]),
-    # FP32: fused multiply-adds, division as reciprocal-multiply and the
-    # hardware reciprocal / square root (about 1 ulp), FP32 denormals flushed;
-    # no reassociation and no finite-math assumptions.  This removes the
-    # range-scaling sequences around every FP32 divide (about 40% of the flux
-    # kernel's VALU instructions).
-    ("fvm_tiled32.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+    # FP32: fused multiply-adds within each source expression
+    # (-ffp-contract=on: the rounding of an expression is fixed by the source,
+    # so every copy of it the compiler makes, peeled or not, gives the same
+    # bits, and cells recomputed by several tiles agree), division as
+    # reciprocal-multiply and the hardware reciprocal / square root (about
+    # 1 ulp), FP32 denormals flushed; no reassociation and no finite-math
+    # assumptions.  This removes the range-scaling sequences around every FP32
+    # divide (about 40% of the flux kernel's VALU instructions).
+    ("fvm_tiled32.hip", ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
                          "-freciprocal-math", "-fapprox-func", "-fgpu-flush-denormals-to-zero"]),
     ("fvm_engine.hip", ["-ffp-contract=off"]),
 ]

@@ -157,6 +157,15 @@ __device__ __forceinline__ int tile_of_block(int bid, int ntiles) {
   return x * q + (x < r ? x : r) + k;
 }
 
+// The addend of a conditional sum `if (in) q += v`, written without a branch:
+// adding -0 leaves every q unchanged (+0 + -0 = +0, -0 + -0 = -0, and q is
+// never denormal under FP32 flush-to-zero), so `q += skip_add(in, v)` gives
+// the same bits as the branch, in the same order.
+template <typename R>
+__device__ __forceinline__ R skip_add(bool in, R v) {
+  return in ? v : R(-0.0);
+}
+
 template <typename R>
 __device__ __forceinline__ Vec4<R> ld(const SVec4<R>& s) {
   return Vec4<R>{s.d, s.du, s.dv, s.de};
@@ -611,9 +620,9 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
         R q0 = R(0.0), q1 = R(0.0), q2 = R(0.0), q3 = R(0.0);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-          if (nbk[k] >= 0) {
-            q0 += v[k].d; q1 += v[k].du; q2 += v[k].dv; q3 += v[k].de;
-          }
+          const bool in = nbk[k] >= 0;
+          q0 += skip_add(in, v[k].d); q1 += skip_add(in, v[k].du);
+          q2 += skip_add(in, v[k].dv); q3 += skip_add(in, v[k].de);
         }
         const R den = R(1.0) + eps * R(nb.w);
         sR1[j] = SVec4<R>{(q0 * eps + ri.d) / den, (q1 * eps + ri.du) / den,
@@ -642,9 +651,9 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
         R q0 = R(0.0), q1 = R(0.0), q2 = R(0.0), q3 = R(0.0);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-          if (lk[k] >= 0) {
-            q0 += v[k].d; q1 += v[k].du; q2 += v[k].dv; q3 += v[k].de;
-          }
+          const bool in = lk[k] >= 0;
+          q0 += skip_add(in, v[k].d); q1 += skip_add(in, v[k].du);
+          q2 += skip_add(in, v[k].dv); q3 += skip_add(in, v[k].de);
         }
         const R den = R(1.0) + eps * R(gc.y);
         const R p0 = (q0 * eps + ri.d) / den, p1 = (q1 * eps + ri.du) / den;
