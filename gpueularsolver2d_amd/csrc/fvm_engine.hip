@@ -198,18 +198,21 @@ class Batch {
   void enqueue_plain_steps(int nsteps);
   void enqueue_steps(int nsteps);
   void build_graph();
+  void drop_graphs();
 
   fvm_scheme_opts o_;
   int device_ = 0;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
-  hipGraphExec_t graph_ = nullptr;
+  hipGraphExec_t graph_[2] = {nullptr, nullptr};  // two instances of the same steps, alternated
   std::vector<Topology> topo_;
   std::vector<TiledMesh> tiled_;
   std::vector<CaseInfo> cases_;
   Arena arena_;
   int tile_cells_ = 0, threads_ = 256, steps_per_graph_ = 20, total_tiles_ = 0;
   int cus_ = 0, blocks_per_cu_ = 0;  // CUs of the device; workgroups of the tile kernel per CU
+  CaseCtl* h_pin_ = nullptr;         // pinned copies of the control blocks, two slots
+  hipEvent_t ev_ctl_[2] = {nullptr, nullptr};
   size_t lds_a_ = 0, lds_b_ = 0, lds_s_ = 0;
   bool fused_ = false;      // one kernel per RK stage
   bool r0_valid_ = false;   // (fused) the residual of stage 0 matches the current state
@@ -338,15 +341,21 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
   // the one-kernel-per-stage graph alternates the step-start buffers, so it
   // holds an even number of steps
   if (fused_) steps_per_graph_ += steps_per_graph_ & 1;
+  HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&h_pin_), 2 * cases_.size() * sizeof(CaseCtl)));
+  for (auto& e : ev_ctl_) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIP_OK(hipStreamSynchronize(stream_));
 }
 
 Batch::~Batch() {
   if (stream_) (void)hipStreamSynchronize(stream_);
-  if (graph_) (void)hipGraphExecDestroy(graph_);
+  for (auto g : graph_)
+    if (g) (void)hipGraphExecDestroy(g);
   if (d_hist_) (void)hipFree(d_hist_);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
+  for (auto e : ev_ctl_)
+    if (e) (void)hipEventDestroy(e);
+  if (h_pin_) (void)hipHostFree(h_pin_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -983,11 +992,18 @@ void Batch::enqueue_steps(int nsteps) {
     enqueue_tiled_steps<float>(nsteps);
 }
 
+void Batch::drop_graphs() {
+  for (auto& g : graph_)
+    if (g) {
+      HIP_OK(hipGraphExecDestroy(g));
+      g = nullptr;
+    }
+}
+
+// Two executable instances of the same captured steps: a replay can be
+// queued while the previous one (of the other instance) still runs.
 void Batch::build_graph() {
-  if (graph_) {
-    HIP_OK(hipGraphExecDestroy(graph_));
-    graph_ = nullptr;
-  }
+  drop_graphs();
   hipGraph_t g;
   HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
   try {
@@ -997,7 +1013,7 @@ void Batch::build_graph() {
     throw;
   }
   HIP_OK(hipStreamEndCapture(stream_, &g));
-  HIP_OK(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+  for (auto& e : graph_) HIP_OK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
   HIP_OK(hipGraphDestroy(g));
 }
 
@@ -1011,10 +1027,7 @@ void Batch::run(int max_steps, double epsl, double* hist, int* steps_done, int* 
     d_hist_ = nullptr;
     hist_stride_ = std::max(max_steps + 1, 2);
     HIP_OK(hipMalloc(&d_hist_, (size_t)ncase * hist_stride_ * sizeof(double)));
-    if (graph_) {
-      HIP_OK(hipGraphExecDestroy(graph_));
-      graph_ = nullptr;
-    }
+    drop_graphs();
   }
   std::vector<int> before(ncase);
   for (int c = 0; c < ncase; c++) {
@@ -1024,7 +1037,7 @@ void Batch::run(int max_steps, double epsl, double* hist, int* steps_done, int* 
     k.epsl = epsl;
     k.active = (k.step < max_steps && k.resd > epsl && k.status == FVM_OK) ? 1 : 0;
   }
-  if (!graph_) build_graph();
+  if (!graph_[0]) build_graph();
   HIP_OK(hipMemcpyAsync(d_ctl_, h_ctl_.data(), ncase * sizeof(CaseCtl), hipMemcpyHostToDevice, stream_));
   if (fused_ && !r0_valid_) {
     if (o_.precision == FVM_FP64)
@@ -1034,21 +1047,40 @@ void Batch::run(int max_steps, double epsl, double* hist, int* steps_done, int* 
     r0_valid_ = true;
   }
   HIP_OK(hipEventRecord(ev0_, stream_));
-  for (;;) {
-    int min_step = max_steps, any = 0;
-    for (int c = 0; c < ncase; c++)
-      if (h_ctl_[c].active) {
-        any = 1;
-        min_step = std::min(min_step, h_ctl_[c].step);
+  // Every active case advances one step per step launched.  Graph replays
+  // are queued one ahead, alternating the two instances: while replay i runs,
+  // the host reads the control blocks copied after replay i - 1 and stops
+  // queueing once no case is active (a replay queued after the last case
+  // stopped only runs launches that return at once).  The steps left over
+  // run without a graph.
+  auto min_active_step = [&](const CaseCtl* c) {
+    int m = -1;
+    for (int i = 0; i < ncase; i++)
+      if (c[i].active) m = m < 0 ? c[i].step : std::min(m, c[i].step);
+    return m;
+  };
+  const int m0 = min_active_step(h_ctl_.data());
+  if (m0 >= 0) {
+    const int remaining = max_steps - m0;
+    const int nfull = remaining / steps_per_graph_, rest = remaining % steps_per_graph_;
+    bool any = true;
+    for (int i = 0; i < nfull && any; i++) {
+      HIP_OK(hipGraphLaunch(graph_[i & 1], stream_));
+      CaseCtl* slot = h_pin_ + (size_t)(i & 1) * ncase;
+      HIP_OK(hipMemcpyAsync(slot, d_ctl_, ncase * sizeof(CaseCtl), hipMemcpyDeviceToHost, stream_));
+      HIP_OK(hipEventRecord(ev_ctl_[i & 1], stream_));
+      if (i > 0) {
+        HIP_OK(hipEventSynchronize(ev_ctl_[(i - 1) & 1]));
+        any = min_active_step(h_pin_ + (size_t)((i - 1) & 1) * ncase) >= 0;
       }
-    if (!any) break;
-    const int remaining = max_steps - min_step;
-    if (remaining >= steps_per_graph_)
-      HIP_OK(hipGraphLaunch(graph_, stream_));
-    else
-      enqueue_steps(remaining);
+    }
     HIP_OK(hipMemcpyAsync(h_ctl_.data(), d_ctl_, ncase * sizeof(CaseCtl), hipMemcpyDeviceToHost, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
+    if (rest > 0 && min_active_step(h_ctl_.data()) >= 0) {
+      enqueue_steps(rest);
+      HIP_OK(hipMemcpyAsync(h_ctl_.data(), d_ctl_, ncase * sizeof(CaseCtl), hipMemcpyDeviceToHost, stream_));
+      HIP_OK(hipStreamSynchronize(stream_));
+    }
   }
   HIP_OK(hipEventRecord(ev1_, stream_));
   HIP_OK(hipEventSynchronize(ev1_));
