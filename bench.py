@@ -186,7 +186,9 @@ def batch64(steps):
     case_mesh = [i % 4 for i in range(64)]
     case_flow = [flows[(i // 4) % 16] for i in range(64)]
     cells = sum(meshes[m].ncell for m in case_mesh)
+    t0 = time.perf_counter()
     F = Batch(meshes, case_mesh, case_flow, precision="fp32")
+    tset = time.perf_counter() - t0
     F.run(20)
     sync()
     t0 = time.perf_counter()
@@ -203,6 +205,7 @@ def batch64(steps):
     d = np.abs(clF - clA)[ok]
     return dict(value=round(4.0 * done / t / 1e6, 1), unit="Mcsu/s", cases=64, cells=int(cells),
                 mesh_cells=[m.ncell for m in meshes], steps=steps, launch=F.config(),
+                setup_s=round(tset, 2),
                 diverged={str(i): dict(mesh_cells=meshes[case_mesh[i]].ncell, **case_flow[i],
                                        fp32_step=int(nF[i]), fp64_step=int(nA[i]))
                           for i in range(64) if not ok[i]},
@@ -295,7 +298,9 @@ def batch256(steps, workers=16):
     tgen = time.perf_counter() - t0
     case_mesh = list(range(256))
     cells = sum(m.ncell for m in meshes)
+    t0 = time.perf_counter()
     F = Batch(meshes, case_mesh, flows, precision="fp32")
+    tset = time.perf_counter() - t0
     F.run(10)
     sync()
     t0 = time.perf_counter()
@@ -322,7 +327,7 @@ def batch256(steps, workers=16):
                 dcd_max=float(dcd[ok].max()),
                 worst_case=dict(case=worst, mesh_cells=meshes[worst].ncell, **flows[worst],
                                 cl_fp64=float(clA[worst])),
-                mesh_generation_s=round(tgen, 1))
+                mesh_generation_s=round(tgen, 1), setup_s=round(tset, 2))
 
 
 def main():

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fvm_internal.h"
@@ -35,6 +36,39 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 static const double kR = 287.06;              // gpu_fvm.cu:8
 static const double kPI = 3.14159265358979;   // gpu_fvm.cu:9
 static const double kRK[4] = {0.0833, 0.2069, 0.4265, 1.0};  // gpu_fvm.cu:10
+
+// fn(0) .. fn(n-1) on up to 16 host threads (per-mesh pre-processing of a
+// batch; the items are independent).  An Error thrown by any item is
+// rethrown for the lowest failing index, so the outcome does not depend on
+// the thread count.
+template <typename Fn>
+static void parallel_for(int n, Fn fn) {
+  const int nt = std::max(1, std::min({n, 16, (int)std::thread::hardware_concurrency()}));
+  std::vector<Error> err(n);
+  std::vector<char> failed(n, 0);
+  auto work = [&](int w) {
+    for (int i = w; i < n; i += nt) {
+      try {
+        fn(i);
+      } catch (const Error& e) {
+        err[i] = e;
+        failed[i] = 1;
+      } catch (const std::exception& e) {
+        err[i] = Error{FVM_ERR_HIP, e.what()};
+        failed[i] = 1;
+      }
+    }
+  };
+  if (nt == 1) {
+    work(0);
+  } else {
+    std::vector<std::thread> pool;
+    for (int w = 0; w < nt; w++) pool.emplace_back(work, w);
+    for (auto& t : pool) t.join();
+  }
+  for (int i = 0; i < n; i++)
+    if (failed[i]) throw err[i];
+}
 
 // ---------------------------------------------------------------------------
 // finalize_step: per case residual and stop test
@@ -296,7 +330,8 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
       throw Error{FVM_ERR_ARGUMENT, "flow[" + std::to_string(i) + "]: need gamma > 1, mach > 0, t_in > 0, cfl > 0"};
   }
   // host pre-processing first: it validates the meshes and the tile tables
-  for (int m = 0; m < nmesh; m++) topo_.push_back(build_topology(meshes[m]));
+  topo_.resize(nmesh);
+  parallel_for(nmesh, [&](int m) { topo_[m] = build_topology(meshes[m]); });
   for (int i = 0; i < ncase; i++) {
     CaseInfo ci;
     ci.mesh = case_mesh[i];
@@ -326,7 +361,8 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
     else
       tile_cells_ = o.precision == FVM_FP64 ? auto_tile_cells<double>() : auto_tile_cells<float>();
     tiled_.clear();
-    for (int m = 0; m < nmesh; m++) tiled_.push_back(build_tiled(topo_[m], tile_cells_));
+    tiled_.resize(nmesh);
+    parallel_for(nmesh, [&](int m) { tiled_[m] = build_tiled(topo_[m], tile_cells_); });
   }
   HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HIP_OK(hipEventCreate(&ev0_));
@@ -398,7 +434,8 @@ int Batch::auto_tile_cells() {
   const int nmesh = (int)topo_.size();
   auto slots_at = [&](int t) {
     tiled_.clear();
-    for (int m = 0; m < nmesh; m++) tiled_.push_back(build_tiled(topo_[m], t));
+    tiled_.resize(nmesh);
+    parallel_for(nmesh, [&](int m) { tiled_[m] = build_tiled(topo_[m], t); });
     const size_t lds = tile_lds<R>();
     int per_cu = 0;
     if (lds <= 160 * 1024) {
