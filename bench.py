@@ -349,7 +349,7 @@ def main():
                     help="steps of the 64-case batch (0 skips it)")
     ap.add_argument("--batch256-steps", type=int, default=3000,
                     help="steps of the 256-case batch (0 skips it)")
-    ap.add_argument("--cpu-steps", type=int, default=40,
+    ap.add_argument("--cpu-steps", type=int, default=100,
                     help="steps of the one-thread CPU sample")
     ap.add_argument("--cpu-omp-steps", type=int, default=200,
                     help="steps of the all-core CPU sample")
