@@ -154,3 +154,18 @@ def test_epsl_stop_matches_reference_loop(rae3k):
     hist, n, st = s.run(300, epsl=0.3)
     assert n == stop
     assert np.array_equal(s.state(), orc.state())
+
+
+def test_fp64_bitwise_on_reference_7k_mesh():
+    # the reference's second mesh (2822-7k): the default launch shape and the
+    # plain kernels against the oracle over 100 steps of the RAE 2822 case
+    mesh = read_dat(os.path.join(GOLDEN, "2822-7k.dat.gz"))
+    steps = 100
+    orc, hist_o = oracle_run(mesh, steps, 0.729, 2.31, 1.0)
+    Wo = orc.state()
+    for kernels in ("tiled", "plain"):
+        s = Solver(mesh, 0.729, 2.31, cfl=1.0, precision="fp64", kernels=kernels)
+        hist, n, st = s.run(steps)
+        assert st == 0 and n == steps
+        assert np.array_equal(s.state(), Wo), kernels
+        np.testing.assert_allclose(hist[1:], hist_o[1:steps + 1], rtol=1e-12, atol=0)
