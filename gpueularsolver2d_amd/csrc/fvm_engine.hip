@@ -253,7 +253,7 @@ class Batch {
   int max_ring_local_ = 0, max_w_local_ = 0;
   void *d_res_b_ = nullptr, *d_dt_b_ = nullptr;
   int* d_rings_ = nullptr;
-  int4* d_lnb_ = nullptr;
+  int2* d_lnb_ = nullptr;
   // control
   CaseCtl* d_ctl_ = nullptr;
   std::vector<CaseCtl> h_ctl_;
@@ -265,7 +265,8 @@ class Batch {
   void* d_flow_ = nullptr;
   FlowT<double>* d_flow64_ = nullptr;
   void *d_cg_ = nullptr, *d_edges_ = nullptr;
-  int4 *d_nbr_ = nullptr, *d_slot_ = nullptr, *d_nloc_ = nullptr;
+  int4 *d_nbr_ = nullptr, *d_nloc_ = nullptr;
+  int2* d_slot_ = nullptr;
   int* d_halo_ = nullptr;
   TileDesc* d_tiles_ = nullptr;
   void *d_w0_ = nullptr, *d_wc_ = nullptr, *d_res_ = nullptr, *d_dt_ = nullptr;
@@ -425,9 +426,10 @@ int Batch::max_local() const {
 // number of waves that kMaxTile-cell tiles need.  Both bounds scale with the
 // workgroup size (0.25 and 0.625 cells per thread: one pass of the threads
 // covers a tile and its rings).  The slots per CU come from the occupancy of
-// the tile kernel (registers and LDS) at kMaxTile; smaller tiles need less
-// LDS and may fit more workgroups, so the choice is checked once at the
-// chosen size and redone with the larger slot count if it grew.
+// the tile kernel (registers and LDS) at kMaxTile.  The chosen, smaller tile
+// may fit more workgroups per CU than that; refilling those extra slots with
+// still smaller tiles was measured to lose (the rings and the idle threads
+// of a smaller tile cost more than the fuller wave gains).
 template <typename R>
 int Batch::auto_tile_cells() {
   const int kMinTile = threads_ / 4, kMaxTile = threads_ * 5 / 8;
@@ -465,12 +467,7 @@ int Batch::auto_tile_cells() {
   const long slots = slots_at(kMaxTile);
   if (slots < 1) return kMaxTile;
   const long waves = (ntiles(kMaxTile) + slots - 1) / slots;
-  const int t = fill(slots, waves);
-  const long slots_t = slots_at(t);
-  if (slots_t <= slots) return t;
-  // a smaller tile keeps at least slots_t slots; a larger one might not
-  const int t2 = fill(slots_t, (ntiles(kMaxTile) + slots_t - 1) / slots_t);
-  return t2 < t ? t2 : t;
+  return fill(slots, waves);
 }
 
 template <typename R>
@@ -485,6 +482,9 @@ void Batch::setup_tiled() {
     max_ring_local_ = std::max(max_ring_local_, t.max_ring_local);
     max_w_local_ = std::max(max_w_local_, t.max_w_local);
   }
+  // gather slots (2 * edge + side) and tile-local ids travel as 16-bit fields
+  if (2 * ME + 1 > 32767 || max_ring_local_ > 32767)
+    throw Error{FVM_ERR_ARGUMENT, "tile_cells too large for 16-bit tile-local indices"};
   lds_a_ = flux_tile_lds<R>(ML, ME);
   lds_b_ = update_tile_lds<R>(ML);
   const int SL = stage_slope_stride(ML);
@@ -560,7 +560,7 @@ void Batch::setup_tiled() {
     cg = arena_.take<CellGeom<R>>(ncell_all);
     d_cg_ = cg;
     d_nbr_ = arena_.take<int4>(ncell_all);
-    d_slot_ = arena_.take<int4>(ncell_all);
+    d_slot_ = arena_.take<int2>(ncell_all);
     d_nloc_ = arena_.take<int4>(ncell_all);
     d_halo_ = arena_.take<int>(nhalo_all);
     edges = arena_.take<EdgeRec<R>>(nedge_all);
@@ -575,7 +575,7 @@ void Batch::setup_tiled() {
       d_res_b_ = arena_.take<SVec4<R>>(ncell_state);
       d_dt_b_ = arena_.take<R>(ncell_state);
       d_rings_ = arena_.take<int>(nring_all);
-      d_lnb_ = arena_.take<int4>(nlnb_all);
+      d_lnb_ = arena_.take<int2>(nlnb_all);
     }
     d_partial_ = arena_.take<double>(tiles.size());
     d_tile_min_ = arena_.take<R>(tiles.size());
@@ -603,19 +603,20 @@ void Batch::setup_tiled() {
     hflow64[c] = cases_[c].f64;
   }
   std::vector<CellGeom<R>> hcg(ncell_all);
-  std::vector<int4> hnbr(ncell_all), hslot(ncell_all), hnloc(ncell_all);
+  std::vector<int4> hnbr(ncell_all), hnloc(ncell_all);
+  std::vector<int2> hslot(ncell_all);
   std::vector<int> hhalo(nhalo_all);
   std::vector<EdgeRec<R>> hedge(nedge_all);
   std::vector<int> hwcell(nwall_all);
   std::vector<double> hwvx(nwall_all), hwvy(nwall_all), hwvn(nwall_all);
   std::vector<int> hrings(fused_ ? nring_all : 0);
-  std::vector<int4> hlnb(fused_ ? nlnb_all : 0);
+  std::vector<int2> hlnb(fused_ ? nlnb_all : 0);
   for (int m = 0; m < nmesh; m++) {
     const TiledMesh& T = tiled_[m];
     if (fused_) {
       std::copy(T.rings.begin(), T.rings.end(), hrings.begin() + ring0[m]);
       for (size_t q = 0; q < T.lnb.size() / 3; q++)
-        hlnb[lnb0[m] + q] = make_int4(T.lnb[3 * q], T.lnb[3 * q + 1], T.lnb[3 * q + 2], 0);
+        hlnb[lnb0[m] + q] = pack_tri16(T.lnb[3 * q], T.lnb[3 * q + 1], T.lnb[3 * q + 2]);
     }
     for (int i = 0; i < T.ncell; i++) {
       CellGeom<R>& g = hcg[geo0[m] + i];
@@ -626,7 +627,7 @@ void Batch::setup_tiled() {
       g.inv_area = (R)T.inv_area[i];
       g.area = (R)T.area[i];
       hnbr[geo0[m] + i] = make_int4(T.nbr[3 * i], T.nbr[3 * i + 1], T.nbr[3 * i + 2], T.clog[i]);
-      hslot[geo0[m] + i] = make_int4(T.slot[3 * i], T.slot[3 * i + 1], T.slot[3 * i + 2], 0);
+      hslot[geo0[m] + i] = pack_tri16(T.slot[3 * i], T.slot[3 * i + 1], T.slot[3 * i + 2]);
       hnloc[geo0[m] + i] = make_int4(T.nloc[3 * i], T.nloc[3 * i + 1], T.nloc[3 * i + 2], 0);
     }
     std::copy(T.halo.begin(), T.halo.end(), hhalo.begin() + halo0[m]);

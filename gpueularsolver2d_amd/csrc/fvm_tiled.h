@@ -50,6 +50,18 @@ struct alignas(16) SVec4 {
   R d, du, dv, de;
 };
 
+// Three signed 16-bit values in 8 bytes (x: bits 0-15 and 16-31, y: bits
+// 0-15): the gather slots and tile-local neighbour ids of a cell, whose
+// ranges the host checks.  Read every launch, so the narrower record is HBM
+// bandwidth saved.
+__host__ __device__ inline int2 pack_tri16(int a, int b, int c) {
+  const unsigned lo = ((unsigned)a & 0xffffu) | (((unsigned)b & 0xffffu) << 16);
+  return make_int2((int)lo, (int)((unsigned)c & 0xffffu));
+}
+__device__ __forceinline__ int4 unpack_tri16(int2 v) {
+  return make_int4((v.x << 16) >> 16, v.x >> 16, (v.y << 16) >> 16, 0);
+}
+
 // Per tile edge: unit normal (n = (vy, vx)/|v|, the reference's convention),
 // length, reconstruction offsets mid - centroid of both cells, and the
 // tile-local indices of the two cells (right: -1 far field, -2 wall).
@@ -116,7 +128,7 @@ struct TiledArgs {
   const FlowT<R>* flow;       // per case
   const CellGeom<R>* cg;      // per mesh cell (all meshes concatenated)
   const int4* nbr;            // per mesh cell: neighbours (new ids, -1 far, -2 wall), .w = CLog
-  const int4* slot;           // per mesh cell: 2*slot + side for the three local edges
+  const int2* slot;           // per mesh cell: 2*slot + side of the three local edges (Tri16)
   const int4* nloc;           // per mesh cell: tile-local index of interior neighbours or -1
   const int* halo;
   const EdgeRec<R>* edges;
@@ -138,7 +150,7 @@ struct TiledArgs {
   const SVec4<R>* r_in;       // residual of the previous stage
   const R* dt_in;             // time step of the previous stage
   const int* rings;           // rings 1..3 of every tile
-  const int4* lnb;            // tile-local neighbours of tile + rings 1..2
+  const int2* lnb;            // tile-local neighbours of tile + rings 1..2 (Tri16)
   int apply_update;           // 0 only for the first launch of a run
   int close_step;             // 1: the update closes a time step (new W0 + partial)
   int max_ring_local, max_w_local;
@@ -347,12 +359,12 @@ __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc
 template <typename R>
 __device__ __forceinline__ R gather_cells(const TiledArgs<R>& a, const TileDesc& d, const FlowT<R>& f,
                                           const SVec4<R>* sF, const R* sS, SVec4<R>* Rout, R* dtout,
-                                          const int4& sl0, R area0) {
+                                          const int2& sl0, R area0) {
   const int s0 = threadIdx.x;
   R tmin = R(3.0e38);
   for (int i = threadIdx.x; i < d.ncell; i += blockDim.x) {
     const int g = d.cell0 + i;
-    const int4 sl = (i == s0) ? sl0 : a.slot[d.geo0 + g];
+    const int4 sl = unpack_tri16((i == s0) ? sl0 : a.slot[d.geo0 + g]);
     const int slk[3] = {sl.x, sl.y, sl.z};
     R T = 0;
     R r0 = R(0.0), r1 = R(0.0), r2 = R(0.0), r3 = R(0.0);
@@ -419,7 +431,7 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
   const int s0 = threadIdx.x;
   EdgeRec<R> e0{};
   if (s0 < d.nedge) e0 = a.edges[d.edge0 + s0];
-  int4 sl0 = make_int4(0, 0, 0, 0);
+  int2 sl0 = make_int2(0, 0);
   R area0 = R(0);
   if (s0 < nt) {
     sl0 = a.slot[d.geo0 + d.cell0 + s0];
@@ -587,14 +599,14 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   const int cur = cc.w0_cur ^ a.parity;
   const SVec4<R>* W0 = (cur ? a.w0b : a.w0a) + d.st0;
   SVec4<R>* W0n = (cur ? a.w0a : a.w0b) + d.st0;
-  const int4* lnb = a.lnb + d.lnb0;
+  const int2* lnb = a.lnb + d.lnb0;
   const CellGeom<R>* cg = a.cg + d.geo0;
   auto gid = [&](int j) { return j < nt ? d.cell0 + j : a.rings[d.ring0 + j - nt]; };
 
   const int s0 = threadIdx.x;
   EdgeRec<R> e0{};
   if (s0 < d.nedge) e0 = a.edges[d.edge0 + s0];
-  int4 sl0 = make_int4(0, 0, 0, 0);
+  int2 sl0 = make_int2(0, 0);
   R area0 = R(0);
   if (s0 < nt) {
     sl0 = a.slot[d.geo0 + d.cell0 + s0];
@@ -643,7 +655,7 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
       const SVec4<R> w0 = W0[g];
       SVec4<R> ri = sR[j];
       if (a.smoothing) {
-        const int4 ln = lnb[j];
+        const int4 ln = unpack_tri16(lnb[j]);
         const int lk[3] = {ln.x, ln.y, ln.z};
         SVec4<R> v[3];
 #pragma unroll
@@ -687,7 +699,7 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   // 3. gradient and bounds of the tile and ring 1 from the updated state
   if (a.order >= 2) {
     for (int j = threadIdx.x; j < NG; j += blockDim.x) {
-      const int4 ln = lnb[j];
+      const int4 ln = unpack_tri16(lnb[j]);
       const CellGeom<R> cgj = cg[sGC[j].x];
       const int nbk[3] = {ln.x, ln.y, ln.z};
       Vec4<R> nbv[3];
