@@ -250,7 +250,7 @@ class Batch {
   size_t lds_a_ = 0, lds_b_ = 0, lds_s_ = 0;
   bool fused_ = false;      // one kernel per RK stage
   bool r0_valid_ = false;   // (fused) the residual of stage 0 matches the current state
-  int max_ring_local_ = 0, max_w_local_ = 0;
+  int max_r4_local_ = 0, max_ring_local_ = 0, max_w_local_ = 0;
   void *d_res_b_ = nullptr, *d_dt_b_ = nullptr;
   int* d_rings_ = nullptr;
   int2* d_lnb_ = nullptr;
@@ -399,15 +399,16 @@ Batch::~Batch() {
 // LDS per workgroup of the kernels this batch launches, for its tile tables.
 template <typename R>
 size_t Batch::tile_lds() const {
-  int ML = 0, ME = 0, MR = 0, MW = 0;
+  int ML = 0, ME = 0, M4 = 0, MR = 0, MW = 0;
   for (auto& t : tiled_) {
     ML = std::max(ML, t.max_local);
     ME = std::max(ME, t.max_edges);
+    M4 = std::max(M4, t.max_r4_local);
     MR = std::max(MR, t.max_ring_local);
     MW = std::max(MW, t.max_w_local);
   }
   const int SL = stage_slope_stride(ML);
-  return fused_ ? stage_tile_lds<R>(MR, MW, SL > 0 ? SL : ML, ME)
+  return fused_ ? stage_tile_lds<R>(M4, MR, MW, SL > 0 ? SL : ML, ME)
                 : std::max(flux_tile_lds<R>(ML, ME), update_tile_lds<R>(ML));
 }
 
@@ -479,16 +480,17 @@ void Batch::setup_tiled() {
     ME = std::max(ME, t.max_edges);
   }
   for (auto& t : tiled_) {
+    max_r4_local_ = std::max(max_r4_local_, t.max_r4_local);
     max_ring_local_ = std::max(max_ring_local_, t.max_ring_local);
     max_w_local_ = std::max(max_w_local_, t.max_w_local);
   }
   // gather slots (2 * edge + side) and tile-local ids travel as 16-bit fields
-  if (2 * ME + 1 > 32767 || max_ring_local_ > 32767)
+  if (2 * ME + 1 > 32767 || max_r4_local_ > 32767)
     throw Error{FVM_ERR_ARGUMENT, "tile_cells too large for 16-bit tile-local indices"};
   lds_a_ = flux_tile_lds<R>(ML, ME);
   lds_b_ = update_tile_lds<R>(ML);
   const int SL = stage_slope_stride(ML);
-  lds_s_ = stage_tile_lds<R>(max_ring_local_, max_w_local_, SL > 0 ? SL : ML, ME);
+  lds_s_ = stage_tile_lds<R>(max_r4_local_, max_ring_local_, max_w_local_, SL > 0 ? SL : ML, ME);
   const size_t lds_max = std::max(lds_a_, fused_ ? lds_s_ : lds_b_);
   if (lds_max > 160 * 1024)
     throw Error{FVM_ERR_ARGUMENT, "tile_cells too large: the tile kernels need " +
@@ -540,6 +542,7 @@ void Batch::setup_tiled() {
       d.ring0 = ring0[m] + T.tile_ring0[k];
       d.n2 = T.ring_n2[k];
       d.n3 = T.ring_n3[k];
+      d.n4 = T.ring_n4[k];
       d.lnb0 = lnb0[m] + T.tile_lnb0[k];
       tiles.push_back(d);
     }
@@ -868,6 +871,7 @@ TiledArgs<R> Batch::tiled_args() {
   a.w0b = static_cast<SVec4<R>*>(d_wc_);
   a.rings = d_rings_;
   a.lnb = d_lnb_;
+  a.max_r4_local = max_r4_local_;
   a.max_ring_local = max_ring_local_;
   a.max_w_local = max_w_local_;
   a.hist = d_hist_;

@@ -69,14 +69,16 @@ struct TiledMesh {
   std::vector<double> e_nx, e_ny, e_vn, e_vx, e_vy, e_rlx, e_rly, e_rrx, e_rry;
   int max_local = 0, max_edges = 0;  // largest tile-local cell and edge counts
   // Rings for the one-kernel-per-stage path: ring 1 is the halo above, ring 2
-  // the interior neighbours of ring 1 outside the tile and ring 1, ring 3
-  // likewise.  The tile-local index space is [tile | ring 1 | ring 2 | ring 3].
-  std::vector<int> tile_ring0;    // [ntile+1] into rings (rings 1..3 of each tile)
-  std::vector<int> ring_n1, ring_n2, ring_n3;  // [ntile]
-  std::vector<int> rings;         // new ids of ring 1, 2, 3 cells, per tile
-  std::vector<int> tile_lnb0;     // [ntile+1] into lnb (one entry per tile/ring-0..2 cell)
+  // the interior neighbours of ring 1 outside the tile and ring 1, rings 3 and
+  // 4 likewise.  The tile-local index space is
+  // [tile | ring 1 | ring 2 | ring 3 | ring 4].
+  std::vector<int> tile_ring0;    // [ntile+1] into rings (rings 1..4 of each tile)
+  std::vector<int> ring_n1, ring_n2, ring_n3, ring_n4;  // [ntile]
+  std::vector<int> rings;         // new ids of ring 1..4 cells, per tile
+  std::vector<int> tile_lnb0;     // [ntile+1] into lnb (one entry per tile/ring-1..3 cell)
   std::vector<int> lnb;           // 3 per entry: tile-local index of each interior neighbour,
                                   // -1 far field, -2 wall
+  int max_r4_local = 0;           // largest tile + rings 1..4 count
   int max_ring_local = 0;         // largest tile + rings 1..3 count
   int max_w_local = 0;            // largest tile + rings 1..2 count
   // wall edges in wall-id order (forces)
@@ -104,7 +106,7 @@ struct TileDesc {               // one (case, mesh tile) pair of a batch
   int cell0, ncell;             // mesh-local new ids owned by the tile
   int halo0, nhalo;             // range in the halo array (global)
   int edge0, nedge;             // range in the edge record array (global)
-  int ring0, n2, n3;            // rings 1..3 list offset (global); ring sizes 2 and 3
+  int ring0, n2, n3, n4;        // rings 1..4 list offset (global); ring sizes 2, 3 and 4
   int lnb0;                     // local-neighbour table offset (global)
 };
 

@@ -306,12 +306,13 @@ TiledMesh build_tiled(const Topology& t, int tile_cells) {
   T.tile_halo0[ntile] = (int)T.halo.size();
   T.tile_edge0[ntile] = (int)T.e_l.size();
 
-  // rings 2 and 3 and the tile-local neighbour table for tile + rings 1..2
+  // rings 2..4 and the tile-local neighbour table for tile + rings 1..3
   T.tile_ring0.resize(ntile + 1);
   T.tile_lnb0.resize(ntile + 1);
   T.ring_n1.resize(ntile);
   T.ring_n2.resize(ntile);
   T.ring_n3.resize(ntile);
+  T.ring_n4.resize(ntile);
   std::vector<int> mark(C, -1);
   for (int k = 0; k < ntile; k++) {
     const int c0 = T.tile_cell0[k], c1 = T.tile_cell0[k + 1], nt = c1 - c0;
@@ -328,8 +329,8 @@ TiledMesh build_tiled(const Topology& t, int tile_cells) {
       local.push_back(T.halo[q]);
     }
     int begin = nt, end = (int)local.size();
-    int ring_sizes[3] = {end - begin, 0, 0};
-    for (int r = 1; r <= 2; r++) {  // build rings 2 and 3 from rings 1 and 2
+    int ring_sizes[4] = {end - begin, 0, 0, 0};
+    for (int r = 1; r <= 3; r++) {  // build rings 2..4 from rings 1..3
       std::vector<int> nxt;
       for (int j = begin; j < end; j++)
         for (int q = 0; q < 3; q++) {
@@ -349,23 +350,27 @@ TiledMesh build_tiled(const Topology& t, int tile_cells) {
     T.ring_n1[k] = ring_sizes[0];
     T.ring_n2[k] = ring_sizes[1];
     T.ring_n3[k] = ring_sizes[2];
+    T.ring_n4[k] = ring_sizes[3];
     T.rings.insert(T.rings.end(), local.begin() + nt, local.end());
     const int nw = nt + ring_sizes[0] + ring_sizes[1];
-    for (int j = 0; j < nw; j++)
+    const int nr = nw + ring_sizes[2];
+    for (int j = 0; j < nr; j++)
       for (int q = 0; q < 3; q++) {
         const int nb = T.nbr[3 * local[j] + q];
         T.lnb.push_back(nb >= 0 ? mark[nb] : nb);
       }
-    T.max_ring_local = std::max(T.max_ring_local, (int)local.size());
+    T.max_r4_local = std::max(T.max_r4_local, (int)local.size());
+    T.max_ring_local = std::max(T.max_ring_local, nr);
     T.max_w_local = std::max(T.max_w_local, nw);
-    // every interior neighbour of tile + rings 1..2 is local, and those of
-    // tile + ring 1 (the gradient cells) lie in tile + rings 1..2; check it
+    // every interior neighbour of tile + rings 1..3 is local, those of
+    // tile + rings 1..2 (smoothing sweep 2) lie in tile + rings 1..3 and those
+    // of tile + ring 1 (the gradient cells) in tile + rings 1..2; check it
     const int ng = nt + ring_sizes[0];
-    for (int j = 0; j < nw; j++)
+    for (int j = 0; j < nr; j++)
       for (int q = 0; q < 3; q++) {
         const int v = T.lnb[(size_t)3 * (T.tile_lnb0[k] + j) + q];
         if (v < -2 || v >= (int)local.size() || (T.nbr[3 * local[j] + q] >= 0 && v < 0) ||
-            (j < ng && v >= nw) || (j < nt && v >= ng))
+            (j < nw && v >= nr) || (j < ng && v >= nw) || (j < nt && v >= ng))
           throw Error{FVM_ERR_MESH, "internal: ring table check failed at tile " + std::to_string(k)};
       }
     for (int id : local) mark[id] = -1;

@@ -5,13 +5,15 @@
 // of the previous stage is recomputed for the tile and two rings of
 // neighbours (so the updated state never leaves LDS), then the gradient,
 // flux and gather of this stage run as in kernel A below.  In order:
-//   1. smoothing sweep 1 of the previous stage's residual on the tile and
-//      rings 1..3 (kernel_Smooth1, gpu_solver.cu:1635-1673);
-//   2. sweep 2, selection and W = W0 - a dt R / A on the tile and rings 1..2
+//   1. the previous stage's residual of the tile and rings 1..4 into LDS,
+//      one load per cell;
+//   2. smoothing sweep 1 on the tile and rings 1..3, gathering from LDS
+//      (kernel_Smooth1, gpu_solver.cu:1635-1673);
+//   3. sweep 2, selection and W = W0 - a dt R / A on the tile and rings 1..2
 //      (kernel_Smooth2 :1675-1720, kernel_RKStep :938-966); in the stage that
 //      closes a time step the tile's new W0 and its FP64 partial of
 //      (rho - rho0)^2 (kernel_drou :968-1020) are written;
-//   3.-5. gradient, flux and gather as in kernel A.
+//   4.-6. gradient, flux and gather as in kernel A.
 // Cells of the rings are updated by several tiles from identical inputs, so
 // every tile sees bitwise the same values.
 //
@@ -149,11 +151,11 @@ struct TiledArgs {
   SVec4<R>* w0b;
   const SVec4<R>* r_in;       // residual of the previous stage
   const R* dt_in;             // time step of the previous stage
-  const int* rings;           // rings 1..3 of every tile
-  const int2* lnb;            // tile-local neighbours of tile + rings 1..2 (Tri16)
+  const int* rings;           // rings 1..4 of every tile
+  const int2* lnb;            // tile-local neighbours of tile + rings 1..3 (Tri16)
   int apply_update;           // 0 only for the first launch of a run
   int close_step;             // 1: the update closes a time step (new W0 + partial)
-  int max_ring_local, max_w_local;
+  int max_r4_local, max_ring_local, max_w_local;
   int parity;                 // step parity within the run: step-start buffer = w0_cur ^ parity
   int close_prev;             // 1: the workgroup of each case's first tile closes the previous step
   double* hist;               // residual history, hist_stride per case
@@ -561,15 +563,16 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
 
 // --------------------------------------------------------------------------
 // One kernel per RK stage (see the header comment).
-// LDS: W[MW] | GC[MW2] | union { R1[MR] | R[MW] ;  grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME] }
-// (ML: the slope row stride, S or max_local)
-// GC holds each entry's cell id and CLog, read in step 1 and reused by
-// steps 2-3 instead of the ring table.  The residual buffers of steps 1-2 are
-// dead once W is complete, so the gradient and flux buffers of steps 3-5
-// reuse their space (a smaller footprint lets more workgroups share a CU).
-// Each item issues its independent global loads together (neighbours of a
-// boundary side read the cell itself and are not added), so one item waits
-// for at most three dependent memory round trips.
+// LDS: W[MW] | gid[MW4] | union { R[M4] | R1[MR] ;  grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME] }
+// (ML: the slope row stride, S or max_local; MW4: MW rounded up to 4)
+// R holds the previous stage's residual of the tile and rings 1..4, one load
+// per cell, so that smoothing sweep 1 gathers from LDS through the 16-bit
+// local neighbour table; a cell's CLog (its count of interior neighbours) is
+// read off the same entry.  gid holds the cell id of each entry of the tile
+// and rings 1..2, read by steps 3-4 instead of the ring table.  The residual
+// buffers of steps 1-3 are dead once W is complete, so the gradient and flux
+// buffers of steps 4-6 reuse their space (a smaller footprint lets more
+// workgroups share a CU).
 // --------------------------------------------------------------------------
 // S > 0: the slope rows are S apart (S >= max_local), a compile-time stride,
 // so the 16 LDS accesses per cell share one address register with immediate
@@ -584,15 +587,15 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   // the cases still iterating
   if (a.apply_update ? !cc.active : cc.status != FVM_OK) return;
   const FlowT<R> f = a.flow[d.icase];
-  const int nt = d.ncell, NG = nt + d.nhalo, NW = NG + d.n2, NR = NW + d.n3;
-  const int MR = a.max_ring_local, MW = a.max_w_local, ME = a.max_edges;
+  const int nt = d.ncell, NG = nt + d.nhalo, NW = NG + d.n2, NR = NW + d.n3, N4 = NR + d.n4;
+  const int M4 = a.max_r4_local, MW = a.max_w_local, ME = a.max_edges;
   const int ML = S > 0 ? S : a.max_local;
-  const int MW2 = (MW + 1) & ~1;
+  const int MW4 = (MW + 3) & ~3;
   SVec4<R>* sW = reinterpret_cast<SVec4<R>*>(smem);
-  int2* sGC = reinterpret_cast<int2*>(sW + MW);
-  SVec4<R>* sR1 = reinterpret_cast<SVec4<R>*>(sGC + MW2);
-  SVec4<R>* sR = sR1 + MR;
-  R* sG = reinterpret_cast<R*>(sR1);
+  int* sGid = reinterpret_cast<int*>(sW + MW);
+  SVec4<R>* sR = reinterpret_cast<SVec4<R>*>(sGid + MW4);
+  SVec4<R>* sR1 = sR + M4;
+  R* sG = reinterpret_cast<R*>(sR);
   R* sB = sG + 8 * ML;
   SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
   R* sS = reinterpret_cast<R*>(sF + ME);
@@ -613,61 +616,77 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     area0 = cg[d.cell0 + s0].area;
   }
 
+  // this thread's first local-neighbour entry: read by sweep 1, reused by
+  // sweep 2 and the gradient (a thread takes the same entry in each)
+  int2 lp0 = make_int2(0, 0);
+  const bool have_lp0 = a.apply_update && a.smoothing && s0 < NR;
   double acc = 0.0;
   if (a.apply_update) {
     const SVec4<R>* Rin = a.r_in + d.st0;
     const R eps = f.eps;
-    // 1. smoothing sweep 1 on the tile and rings 1..3 (own residual kept for
-    //    the tile and rings 1..2)
-    const int n1 = a.smoothing ? NR : NW;
+    // 1. the previous stage's residual of the tile and rings 1..4 (1..2
+    //    without smoothing)
+    const int n1 = a.smoothing ? N4 : NW;
     for (int j = threadIdx.x; j < n1; j += blockDim.x) {
       const int g = gid(j);
-      const SVec4<R> ri = Rin[g];
-      if (a.smoothing) {
-        const int4 nb = a.nbr[d.geo0 + g];
-        const int nbk[3] = {nb.x, nb.y, nb.z};
-        SVec4<R> v[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) v[k] = Rin[nbk[k] >= 0 ? nbk[k] : g];
-        R q0 = R(0.0), q1 = R(0.0), q2 = R(0.0), q3 = R(0.0);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          const bool in = nbk[k] >= 0;
-          q0 += skip_add(in, v[k].d); q1 += skip_add(in, v[k].du);
-          q2 += skip_add(in, v[k].dv); q3 += skip_add(in, v[k].de);
-        }
-        const R den = R(1.0) + eps * R(nb.w);
-        sR1[j] = SVec4<R>{(q0 * eps + ri.d) / den, (q1 * eps + ri.du) / den,
-                          (q2 * eps + ri.dv) / den, (q3 * eps + ri.de) / den};
-        if (j < NW) sGC[j] = make_int2(g, nb.w);
-      } else {
-        sGC[j] = make_int2(g, 0);
-      }
-      if (j < NW) sR[j] = ri;
+      sR[j] = Rin[g];
+      if (j < NW) sGid[j] = g;
     }
     __syncthreads();
-    // 2. sweep 2, selection and the RK update on the tile and rings 1..2
-    for (int j = threadIdx.x; j < NW; j += blockDim.x) {
-      const int2 gc = sGC[j];
-      const int g = gc.x;
-      const R dtv = a.dt_in[d.st0 + g];
-      const R area = cg[g].area;
-      const SVec4<R> w0 = W0[g];
-      SVec4<R> ri = sR[j];
-      if (a.smoothing) {
-        const int4 ln = unpack_tri16(lnb[j]);
+    // 2. smoothing sweep 1 on the tile and rings 1..3
+    if (a.smoothing) {
+      for (int j = threadIdx.x; j < NR; j += blockDim.x) {
+        const int2 lp = lnb[j];
+        if (j == s0) lp0 = lp;
+        const int4 ln = unpack_tri16(lp);
         const int lk[3] = {ln.x, ln.y, ln.z};
+        const SVec4<R> ri = sR[j];
         SVec4<R> v[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) v[k] = sR1[lk[k] >= 0 ? lk[k] : j];
+        for (int k = 0; k < 3; k++) v[k] = sR[lk[k] >= 0 ? lk[k] : j];
         R q0 = R(0.0), q1 = R(0.0), q2 = R(0.0), q3 = R(0.0);
+        int clog = 0;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
           const bool in = lk[k] >= 0;
           q0 += skip_add(in, v[k].d); q1 += skip_add(in, v[k].du);
           q2 += skip_add(in, v[k].dv); q3 += skip_add(in, v[k].de);
+          clog += in;
         }
-        const R den = R(1.0) + eps * R(gc.y);
+        const R den = R(1.0) + eps * R(clog);
+        sR1[j] = SVec4<R>{(q0 * eps + ri.d) / den, (q1 * eps + ri.du) / den,
+                          (q2 * eps + ri.dv) / den, (q3 * eps + ri.de) / den};
+      }
+      __syncthreads();
+    }
+    // 3. sweep 2, selection and the RK update on the tile and rings 1..2
+    for (int j = threadIdx.x; j < NW; j += blockDim.x) {
+      const int g = sGid[j];
+      const R dtv = a.dt_in[d.st0 + g];
+      const R area = cg[g].area;
+      const SVec4<R> w0 = W0[g];
+      SVec4<R> ri = sR[j];
+      if (a.smoothing) {
+        int2 lp;
+        if (j == s0)
+          lp = lp0;
+        else
+          lp = lnb[j];
+        const int4 ln = unpack_tri16(lp);
+        const int lk[3] = {ln.x, ln.y, ln.z};
+        SVec4<R> v[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) v[k] = sR1[lk[k] >= 0 ? lk[k] : j];
+        R q0 = R(0.0), q1 = R(0.0), q2 = R(0.0), q3 = R(0.0);
+        int clog = 0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          const bool in = lk[k] >= 0;
+          q0 += skip_add(in, v[k].d); q1 += skip_add(in, v[k].du);
+          q2 += skip_add(in, v[k].dv); q3 += skip_add(in, v[k].de);
+          clog += in;
+        }
+        const R den = R(1.0) + eps * R(clog);
         const R p0 = (q0 * eps + ri.d) / den, p1 = (q1 * eps + ri.du) / den;
         const R p2 = (q2 * eps + ri.dv) / den, p3 = (q3 * eps + ri.de) / den;
         if (fabs(ri.d) > fabs(p0)) ri.d = p0;
@@ -692,15 +711,20 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     for (int j = threadIdx.x; j < NW; j += blockDim.x) {
       const int g = gid(j);
       sW[j] = W0[g];
-      sGC[j] = make_int2(g, 0);
+      sGid[j] = g;
     }
   }
   __syncthreads();
-  // 3. gradient and bounds of the tile and ring 1 from the updated state
+  // 4. gradient and bounds of the tile and ring 1 from the updated state
   if (a.order >= 2) {
     for (int j = threadIdx.x; j < NG; j += blockDim.x) {
-      const int4 ln = unpack_tri16(lnb[j]);
-      const CellGeom<R> cgj = cg[sGC[j].x];
+      int2 lp;
+      if (j == s0 && have_lp0)
+        lp = lp0;
+      else
+        lp = lnb[j];
+      const int4 ln = unpack_tri16(lp);
+      const CellGeom<R> cgj = cg[sGid[j]];
       const int nbk[3] = {ln.x, ln.y, ln.z};
       Vec4<R> nbv[3];
 #pragma unroll
@@ -709,10 +733,10 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     }
   }
   __syncthreads();
-  // 4. flux and spectral radius of the tile's edges
+  // 5. flux and spectral radius of the tile's edges
   flux_edges<R>(a, d, f, sW, sG, sB, sF, sS, ML, e0);
   __syncthreads();
-  // 5. ordered gather: this stage's residual and time step
+  // 6. ordered gather: this stage's residual and time step
   gather_cells<R>(a, d, f, sF, sS, a.res, a.dt, sl0, area0);
   if (a.close_step && a.apply_update) {
     const double s = block_sum(acc);
