@@ -23,7 +23,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "autotune_cache.json")
 
-DEFAULT_CANDIDATES = [(0, 128), (0, 192), (0, 256)] + [
+DEFAULT_CANDIDATES = [(0, 128), (0, 192), (0, 256), (0, 384), (0, 512)] + [
     (t, n) for t in (96, 128, 160, 192, 256) for n in (128, 256)]
 
 

@@ -352,11 +352,11 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
   if (o.kernels != FVM_KERNELS_PLAIN) {
     if (o.tile_cells > 8192)
       throw Error{FVM_ERR_ARGUMENT, "opts.tile_cells must be at most 8192"};
-    // default workgroup size from the launch-shape sweep on MI355X: the FP32
-    // stage kernel (70-80 VGPRs) gains from 512-thread tiles (less ring
-    // recomputation per cell, 3 workgroups per CU); the FP64 kernels (~140
-    // VGPRs, 3 waves per SIMD) keep 256
-    threads_ = o.threads > 0 ? o.threads : (fused_ && o.precision == FVM_FP32 ? 512 : 256);
+    // default workgroup size from the launch-shape sweep on MI355X: the stage
+    // kernel gains from 512-thread tiles (less ring recomputation per cell;
+    // FP32 at 64 VGPRs runs 3 workgroups per CU, FP64 at 112 VGPRs 2); the
+    // two-kernel path keeps 256
+    threads_ = o.threads > 0 ? o.threads : (fused_ ? 512 : 256);
     if (o.tile_cells > 0)
       tile_cells_ = o.tile_cells;
     else

@@ -304,15 +304,16 @@ __device__ __forceinline__ void gradient_cell(int j, const Vec4<R>& Wi, const Ce
 
 // Roe flux and spectral radius of every edge of the tile into LDS slots
 // (kernel_Flux2 gpu_solver.cu:651-764 with kernel_LBJinterpolate :496-549,
-// kernel_deltT :385-454).  e0 is this thread's prefetched first record.
-template <typename R>
+// kernel_deltT :385-454).  kPre: e0 is this thread's prefetched first record
+// (otherwise unused).
+template <typename R, bool kPre = true>
 __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc& d,
                                            const FlowT<R>& f, const SVec4<R>* sW, const R* sG,
                                            const R* sB, SVec4<R>* sF, R* sS, int ML,
                                            const EdgeRec<R>& e0) {
   const int s0 = threadIdx.x;
   for (int s = threadIdx.x; s < d.nedge; s += blockDim.x) {
-    const EdgeRec<R> e = (s == s0) ? e0 : a.edges[d.edge0 + s];
+    const EdgeRec<R> e = (kPre && s == s0) ? e0 : a.edges[d.edge0 + s];
     const EdgeView ev = edge_cells(e);
     const R vx = edge_vx(e), vy = edge_vy(e);
     const Vec4<R> WL = ld(sW[ev.l]);
@@ -606,9 +607,13 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   const CellGeom<R>* cg = a.cg + d.geo0;
   auto gid = [&](int j) { return j < nt ? d.cell0 + j : a.rings[d.ring0 + j - nt]; };
 
+  // this thread's first edge record is loaded here, its latency hidden by
+  // the sweeps, in FP32; the 80-byte FP64 record would hold 20 registers
+  // through them and cost a wave per SIMD, so FP64 loads it in the flux loop
+  constexpr bool kPreEdge = sizeof(R) == 4;
   const int s0 = threadIdx.x;
   EdgeRec<R> e0{};
-  if (s0 < d.nedge) e0 = a.edges[d.edge0 + s0];
+  if (kPreEdge && s0 < d.nedge) e0 = a.edges[d.edge0 + s0];
   int2 sl0 = make_int2(0, 0);
   R area0 = R(0);
   if (s0 < nt) {
@@ -734,7 +739,7 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   }
   __syncthreads();
   // 5. flux and spectral radius of the tile's edges
-  flux_edges<R>(a, d, f, sW, sG, sB, sF, sS, ML, e0);
+  flux_edges<R, kPreEdge>(a, d, f, sW, sG, sB, sF, sS, ML, e0);
   __syncthreads();
   // 6. ordered gather: this stage's residual and time step
   gather_cells<R>(a, d, f, sF, sS, a.res, a.dt, sl0, area0);

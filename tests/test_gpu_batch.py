@@ -110,7 +110,7 @@ def test_kernel_times_and_config(meshes):
     s.kernel_times(reps=3)  # timing launches leave the state unchanged
     assert np.array_equal(s.state(), W)
     # library-chosen tile size: the tiles fill whole waves of the CUs' slots
-    for prec, threads in (("fp32", 512), ("fp64", 256)):
+    for prec, threads in (("fp32", 512), ("fp64", 512)):
         c = Solver(meshes[1], precision=prec, **FLOWS[1]).config()
         slots = c["blocks_per_cu"] * c["cus"]
         assert c["threads"] == threads and slots > 0
