@@ -71,24 +71,6 @@ static void parallel_for(int n, Fn fn) {
 }
 
 // ---------------------------------------------------------------------------
-// finalize_step: per case residual and stop test
-// ---------------------------------------------------------------------------
-// Closes the time step of every active case (one workgroup per case, see
-// close_case): after the last stage of each step in the two-kernel and plain
-// paths, and after the last step of each batch of steps in the
-// one-kernel-per-stage path, whose other steps close inside the next step's
-// first launch.  A separate launch: the kernel boundary orders the tiles'
-// partial sums before the close without a device-scope fence, which on MI355X
-// would write back a whole L2 per workgroup.
-constexpr int kFinalizeThreads = 256;
-
-__global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(CaseCtl* ctl, const double* partial,
-                                                                    double* hist, int hist_stride,
-                                                                    int flip_w0) {
-  close_case(ctl[blockIdx.x], partial, hist, hist_stride, blockIdx.x, flip_w0);
-}
-
-// ---------------------------------------------------------------------------
 // Forces: first-order Roe wall flux of the current state, FP64, wall-id order
 // ---------------------------------------------------------------------------
 struct WallTable {
@@ -915,9 +897,8 @@ void Batch::enqueue_fused_steps(int nsteps) {
     }
   }
   if (nsteps > 0) {
-    hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(kFinalizeThreads), 0, stream_, d_ctl_,
-                       d_partial_, d_hist_, hist_stride_, 0);
-    HIP_OK(hipGetLastError());
+    HIP_OK(launch_finalize<R>(d_ctl_, static_cast<double*>(d_partial_), d_hist_, hist_stride_,
+                              (int)cases_.size(), stream_));
   }
 }
 
@@ -948,9 +929,8 @@ void Batch::enqueue_tiled_steps(int nsteps) {
                                        static_cast<R*>(d_dtg_), (int)cases_.size(), stream_));
       HIP_OK(launch_update_tile<R>(a, threads_, lds_b_, stream_));
     }
-    hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(kFinalizeThreads), 0, stream_, d_ctl_,
-                       d_partial_, d_hist_, hist_stride_, 0);
-    HIP_OK(hipGetLastError());
+    HIP_OK(launch_finalize<R>(d_ctl_, static_cast<double*>(d_partial_), d_hist_, hist_stride_,
+                              (int)cases_.size(), stream_));
   }
 }
 
@@ -1015,9 +995,8 @@ void Batch::enqueue_plain_steps(int nsteps) {
       plain_launch_step(plain_[c], d_ctl_, c, d_flow64_, o_.order, o_.smoothing, o_.global_dt,
                         stream_);
     HIP_OK(hipGetLastError());
-    hipLaunchKernelGGL(finalize_kernel, dim3(cases_.size()), dim3(kFinalizeThreads), 0, stream_, d_ctl_,
-                       d_partial_, d_hist_, hist_stride_, 0);
-    HIP_OK(hipGetLastError());
+    HIP_OK(launch_finalize<double>(d_ctl_, static_cast<double*>(d_partial_), d_hist_, hist_stride_,
+                                   (int)cases_.size(), stream_));
   }
 }
 
@@ -1292,7 +1271,7 @@ const char* fvm_status_string(fvm_status s) {
     case FVM_ERR_MESH: return "invalid mesh";
     case FVM_ERR_AREA: return "non-positive cell area";
     case FVM_ERR_HIP: return "HIP runtime error";
-    case FVM_ERR_NAN: return "non-finite residual";
+    case FVM_ERR_NAN: return "NaN residual (the case diverged)";
     case FVM_ERR_NOT_CONVERGED: return "not converged";
     case FVM_ERR_NO_DEVICE: return "no HIP device";
     case FVM_ERR_ALLOC: return "allocation failed";
@@ -1361,7 +1340,7 @@ fvm_status fvm_batch_run(fvm_batch* b, int32_t max_steps, double epsl, double* r
   if (case_status) std::copy(cs.begin(), cs.end(), case_status);
   for (int32_t s : cs)
     if (s != FVM_OK) {
-      fvm::set_last_error("at least one case's residual became non-finite");
+      fvm::set_last_error("at least one case's residual became NaN");
       return (fvm_status)s;
     }
   return FVM_OK;
@@ -1445,7 +1424,7 @@ fvm_status fvm_run(fvm_solver* s, int32_t max_steps, double epsl, double* resid_
     s->b->run(max_steps, epsl, resid_hist, steps_done, &status);
   });
   if (st != FVM_OK) return st;
-  if (status != FVM_OK) fvm::set_last_error("the residual became non-finite");
+  if (status != FVM_OK) fvm::set_last_error("the residual became NaN");
   return (fvm_status)status;
 }
 

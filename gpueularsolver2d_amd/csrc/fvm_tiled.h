@@ -216,9 +216,8 @@ __device__ __forceinline__ void add_pair(double& s, double& c, double s2, double
 // partials t, t + blockDim, ... (Neumaier), a butterfly over each wave
 // combines lane pairs lower lane first (both lanes of a pair then hold bitwise
 // the same value), then thread 0 adds the waves in order.
-// flip_w0: the step's new state went to the other step-start buffer.
 __device__ inline void close_case(CaseCtl& c, const double* partial, double* hist, int hist_stride,
-                                  int icase, int flip_w0) {
+                                  int icase) {
   if (!c.active) return;
   const int n = c.ntile;
   const double* p = partial + c.tile0;
@@ -248,13 +247,35 @@ __device__ inline void close_case(CaseCtl& c, const double* partial, double* his
   const double drou = S + C;
   const int step = c.step + 1;
   c.step = step;
-  if (flip_w0) c.w0_cur ^= 1;
   if (step == 1) c.epsm = drou;
-  const double resd = sqrt((drou + 1.0E-32) / c.epsm);
+  // A step 1 that changed nothing (drou_1 = 0: a steady state such as
+  // freestream on a mesh without walls) gives +inf, as IEEE division does.
+  // It is written out because the FP32 translation unit, whose kernels close
+  // steps too, is built with a reciprocal relaxation that would give NaN.
+  const double num = drou + 1.0E-32;
+  const double resd = c.epsm != 0.0 ? sqrt(num / c.epsm) : (isnan(num) ? num : HUGE_VAL);
   c.resd = resd;
   if (step < hist_stride) hist[(size_t)icase * hist_stride + step] = resd;
-  if (!isfinite(resd)) c.status = FVM_ERR_NAN;
+  // Resd is +inf, not NaN, when step 1 changed nothing (drou = 0, a state
+  // that is already steady, e.g. freestream on a mesh without walls); the
+  // reference's loop goes on then (inf > EPSL), and so does this one
+  if (isnan(resd)) c.status = FVM_ERR_NAN;
   if (!(step < c.max_steps && resd > c.epsl)) c.active = 0;
+}
+
+// The step close as a launch of its own, one workgroup per case: used after
+// the last step of each launch sequence (the other steps close inside the
+// next step's first stage launch) and by the two-kernel and plain paths.  A
+// separate launch orders the tiles' partial sums before the close without a
+// device-scope fence, which on MI355X would write back a whole L2 per
+// workgroup.  It is instantiated in each precision's translation unit (R is
+// otherwise unused) so that every close of a run, in a stage kernel or here,
+// is the same code built with the same floating-point options.
+constexpr int kFinalizeThreads = 256;
+template <typename R>
+__global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(CaseCtl* ctl, const double* partial,
+                                                                   double* hist, int hist_stride) {
+  close_case(ctl[blockIdx.x], partial, hist, hist_stride, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -754,7 +775,7 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   // launch's parity, not from the control block.
   if (a.close_prev && t == cc.tile0) {
     __syncthreads();
-    close_case(a.ctl[d.icase], a.partial, a.hist, a.hist_stride, d.icase, 0);
+    close_case(a.ctl[d.icase], a.partial, a.hist, a.hist_stride, d.icase);
   }
 }
 

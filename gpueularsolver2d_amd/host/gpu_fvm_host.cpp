@@ -197,7 +197,7 @@ int ROE_SOLVER() {
     fvm_destroy(s);
     return fail("fvm_run", st);
   }
-  if (st == FVM_ERR_NAN) std::fprintf(stderr, "warning: the residual became non-finite at step %d\n", step);
+  if (st == FVM_ERR_NAN) std::fprintf(stderr, "warning: the residual became NaN at step %d\n", step);
   if (!opt.quiet)
     for (int i = 1; i <= step; i++) std::printf("%d,%1.20lf\n", i, Resd[i]);
   std::vector<double> rho(Ncell), ru(Ncell), rv(Ncell), re(Ncell);

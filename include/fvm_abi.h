@@ -39,7 +39,7 @@ typedef enum fvm_status {
     FVM_ERR_MESH = 2,          /* node id out of range, non-manifold edge, unreadable file */
     FVM_ERR_AREA = 3,          /* a cell with area <= 0 (clockwise or degenerate) */
     FVM_ERR_HIP = 4,           /* HIP runtime error (fvm_last_error has the text) */
-    FVM_ERR_NAN = 5,           /* non-finite residual: the case diverged */
+    FVM_ERR_NAN = 5,           /* NaN residual: the case diverged */
     FVM_ERR_NOT_CONVERGED = 6, /* reserved: fvm_run reports max_steps exhaustion via steps_done */
     FVM_ERR_NO_DEVICE = 7,     /* no HIP device, or the requested ordinal does not exist */
     FVM_ERR_ALLOC = 8          /* host or device allocation failed */
@@ -125,7 +125,9 @@ fvm_status fvm_create(const fvm_mesh_desc *mesh, const fvm_flow_params *flow,
  * resid_hist (may be NULL) receives Resd[0..steps] and must hold
  * max_steps+1 doubles; resid_hist[0] = 100 as in the reference.
  * steps_done receives the step count reached.  Returns FVM_ERR_NAN if the
- * residual became non-finite. */
+ * residual became NaN (the case diverged).  A residual of +inf is not an
+ * error: the reference's normalisation by step 1 gives it when step 1 changed
+ * nothing, and its loop goes on. */
 fvm_status fvm_run(fvm_solver *s, int32_t max_steps, double epsl, double *resid_hist,
                    int32_t *steps_done);
 

@@ -116,3 +116,20 @@ def test_kernel_times_and_config(meshes):
         assert c["threads"] == threads and slots > 0
         assert threads // 4 <= c["tile_cells"] <= threads * 5 // 8
         assert c["tiles"] == -(-meshes[1].ncell // c["tile_cells"]) and c["tiles"] <= slots
+
+
+def test_results_independent_of_graph_length(meshes):
+    # The graph length decides which steps close inside a stage launch and
+    # which in finalize_kernel; both are the same code built with the
+    # precision's options, so histories and states match bit for bit.
+    for precision in ("fp32", "fp64"):
+        ref = None
+        for spg in (2, 6, 20):
+            s = Solver(meshes[0], precision=precision, steps_per_graph=spg, **FLOWS[1])
+            h, n, st = s.run(50)
+            assert n == 50 and st == 0
+            if ref is None:
+                ref = (h, s.state())
+            else:
+                np.testing.assert_array_equal(h, ref[0])
+                assert np.array_equal(s.state(), ref[1])

@@ -12,6 +12,7 @@ from gpueularsolver2d_amd import Solver, mesh_topology, naca0012_mesh, read_dat
 from gpueularsolver2d_amd.meshgen import Mesh
 from gpueularsolver2d_amd.solver import EXPORTS, FvmError, load_library
 from oracle_ctypes import OracleMesh
+from small_meshes import annulus, square2
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
@@ -39,10 +40,15 @@ def test_status_strings():
     assert L.fvm_status_string(0) == b"ok"
 
 
-@pytest.mark.parametrize("name", ["2822-3k", "2822-7k", "naca3000"])
+@pytest.mark.parametrize("name", ["2822-3k", "2822-7k", "naca3000", "annulus8x1", "annulus24x3",
+                                  "square2"])
 def test_topology_equals_oracle(name):
     if name.startswith("naca"):
         m = naca0012_mesh(3000, seed=1)
+    elif name.startswith("annulus"):
+        m = annulus(*map(int, name[7:].split("x")))
+    elif name == "square2":
+        m = square2()
     else:
         m = read_dat(os.path.join(GOLDEN, name + ".dat.gz"))
     t = mesh_topology(m)
