@@ -384,7 +384,7 @@ def main():
         return 0
     if args.gpus != 1:
         print("bench.py measures one GPU only: the solver has no multi-GPU path "
-              "(independent cases would be split across GPUs by case; see DESIGN.md)",
+              "(independent cases would be split across GPUs by case)",
               file=sys.stderr)
         return 2
 
