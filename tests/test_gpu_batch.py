@@ -133,3 +133,13 @@ def test_results_independent_of_graph_length(meshes):
             else:
                 np.testing.assert_array_equal(h, ref[0])
                 assert np.array_equal(s.state(), ref[1])
+
+
+def test_oversized_tiles_are_argument_errors(meshes):
+    # a tile whose LDS footprint exceeds the CU's 160 KB is rejected at
+    # creation with a status, before any launch
+    for precision, threads in (("fp32", 0), ("fp64", 0), ("fp32", 64)):
+        with pytest.raises(FvmError) as e:
+            Solver(meshes[1], precision=precision, tile_cells=6000, threads=threads, **FLOWS[0])
+        assert e.value.name == "FVM_ERR_ARGUMENT"
+        assert "LDS" in str(e.value) or "too large" in str(e.value)
