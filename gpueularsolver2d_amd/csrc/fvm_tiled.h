@@ -642,10 +642,12 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     area0 = cg[d.cell0 + s0].area;
   }
 
-  // this thread's first local-neighbour entry: read by sweep 1, reused by
-  // sweep 2 and the gradient (a thread takes the same entry in each)
+  // this thread's first local-neighbour entry, read here with the other
+  // prefetches and used by sweeps 1 and 2 and the gradient (a thread takes
+  // the same entry in each)
   int2 lp0 = make_int2(0, 0);
   const bool have_lp0 = a.apply_update && a.smoothing && s0 < NR;
+  if (have_lp0) lp0 = lnb[s0];
   double acc = 0.0;
   if (a.apply_update) {
     const SVec4<R>* Rin = a.r_in + d.st0;
@@ -662,8 +664,11 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     // 2. smoothing sweep 1 on the tile and rings 1..3
     if (a.smoothing) {
       for (int j = threadIdx.x; j < NR; j += blockDim.x) {
-        const int2 lp = lnb[j];
-        if (j == s0) lp0 = lp;
+        int2 lp;
+        if (j == s0)
+          lp = lp0;
+        else
+          lp = lnb[j];
         const int4 ln = unpack_tri16(lp);
         const int lk[3] = {ln.x, ln.y, ln.z};
         const SVec4<R> ri = sR[j];
