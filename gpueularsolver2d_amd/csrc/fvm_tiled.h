@@ -694,7 +694,11 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     for (int j = threadIdx.x; j < NW; j += blockDim.x) {
       const int g = sGid[j];
       const R dtv = a.dt_in[d.st0 + g];
-      const R area = cg[g].area;
+      R area;
+      if (j == s0 && j < nt)
+        area = area0;  // this thread's tile cell, read at kernel start
+      else
+        area = cg[g].area;
       const SVec4<R> w0 = W0[g];
       SVec4<R> ri = sR[j];
       if (a.smoothing) {
