@@ -44,23 +44,83 @@ def test_missing_files_are_errors_not_exit0(driver, tmp_path):
     assert r.returncode == 1
 
 
-@pytest.mark.gpu
-def test_driver_reproduces_reference_residual_file(driver, tmp_path):
-    write_case(str(tmp_path), 500)
-    r = subprocess.run([driver, "--no-smooth", "--resid-every", "10", "--quiet"], cwd=tmp_path,
+@pytest.fixture(scope="module")
+def driver_run(driver, tmp_path_factory):
+    """One end-to-end run of the host program: the reference's mesh and case,
+    no smoothing, 500 steps, residual every 10 steps."""
+    d = tmp_path_factory.mktemp("driver")
+    write_case(str(d), 500)
+    r = subprocess.run([driver, "--no-smooth", "--resid-every", "10", "--quiet"], cwd=d,
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    rows = [l.split() for l in open(tmp_path / "unresidual.plt").read().splitlines()[3:]]
+    return d
+
+
+@pytest.mark.gpu
+def test_driver_reproduces_reference_residual_file(driver_run):
+    rows = [l.split() for l in open(driver_run / "unresidual.plt").read().splitlines()[3:]]
     got = {int(a): float(b) for a, b in rows}
     with open(os.path.join(GOLDEN, "reference_residuals.json")) as f:
         ref = json.load(f)["unresidual_nosmooth"]
     assert len(ref) == 50
     for stepno, val in ref:
         assert abs(got[stepno] - val) < 0.6e-10, (stepno, got[stepno], val)
-    # node averages and wall pressure files are written and finite
-    res = open(tmp_path / "unresult.plt").read().splitlines()
-    assert res[3].startswith("N=1907,E=3558")
-    vals = np.array([[float(v) for v in l.split()] for l in res[6:6 + 1907]])
-    assert np.isfinite(vals).all() and (vals[:, 2] > 0).all()
-    cp = open(tmp_path / "unairfoilpress.plt").read().splitlines()
-    assert len(cp) == 3 + 2 * 128 + 128
+
+
+def reference_postprocessing(mesh, W, gamma=1.4, p_in=101325.0, t_in=288.15, mach=0.729):
+    """Node_Computing and Output's wall pressure (gpu_fvm.cu:71-131, 182-185)
+    restated in numpy: area-weighted cell-to-node averages of rho, rho u,
+    rho v, P, T and A (CalculationPTA, gpu_init.cu:49-61), then u, v and the
+    Mach number per node; -Cp = -2 (p - PIN / (ROUIN VIN^2)) at wall nodes."""
+    R = 287.06
+    x, y, tri = mesh.x, mesh.y, mesh.tri - 1
+    a, b, c = tri[:, 0], tri[:, 1], tri[:, 2]
+    area = 0.5 * ((y[c] - y[a]) * (x[b] - x[a]) - (y[b] - y[a]) * (x[c] - x[a]))
+    rho, ru, rv, re = W.T
+    u, v = ru / rho, rv / rho
+    e = re / rho - 0.5 * (u * u + v * v)
+    P = (gamma - 1.0) * rho * e
+    A = np.sqrt(gamma * P / rho)
+    T = (gamma - 1.0) * e / R
+    n = len(x)
+    acc = np.zeros((n, 8))
+    for q, val in enumerate((rho, ru, rv, re, T, P, A, np.ones_like(rho))):
+        for k in range(3):
+            np.add.at(acc[:, q], tri[:, k], val * area)
+    ra = acc[:, 7]
+    ROU = acc[:, 0] / ra
+    U = acc[:, 1] / ra / ROU
+    V = acc[:, 2] / ra / ROU
+    Tn, Pn, An = acc[:, 4] / ra, acc[:, 5] / ra, acc[:, 6] / ra
+    Mach = np.sqrt(U * U + V * V) / An
+    ain = np.sqrt(gamma * R * t_in)
+    vin = mach * ain
+    rouin = p_in / (R * t_in)
+    cp = -2.0 * (Pn - p_in / (rouin * vin * vin))
+    return np.column_stack([x, y, ROU, U, V, Pn, Tn, Mach]), cp
+
+
+@pytest.mark.gpu
+def test_driver_node_averages_and_wall_pressure(driver_run):
+    # the same case through the library gives the state the program wrote out
+    from gpueularsolver2d_amd import Solver, mesh_topology, read_dat
+    mesh = read_dat(os.path.join(GOLDEN, "2822-3k.dat.gz"))
+    s = Solver(mesh, 0.729, 2.31, cfl=1.0, smoothing=False, precision="fp64")
+    s.run(500)
+    nodes, cp = reference_postprocessing(mesh, s.state())
+    res = open(driver_run / "unresult.plt").read().splitlines()
+    assert res[3].startswith(f"N={mesh.nnode},E={mesh.ncell}")
+    got = np.array([[float(v) for v in l.split()] for l in res[6:6 + mesh.nnode]])
+    np.testing.assert_allclose(got, nodes, rtol=0, atol=1.5e-10)   # 10 printed decimals
+    tri = np.array([[int(v) for v in l.split()] for l in res[6 + mesh.nnode:]])
+    assert np.array_equal(tri, mesh.tri)
+    # wall pressure: both nodes of every wall edge in wall-id order
+    t = mesh_topology(mesh)
+    wall = np.flatnonzero(t["elog"] == 2)
+    wall = wall[np.argsort(t["wallid"][wall])]
+    pts = np.column_stack([t["node1"][wall], t["node2"][wall]]).ravel() - 1
+    lines = open(driver_run / "unairfoilpress.plt").read().splitlines()
+    assert lines[2].startswith(f"ZONE N={2 * len(wall)} E={len(wall)}")
+    got = np.array([[float(v) for v in l.split()] for l in lines[3:3 + 2 * len(wall)]])
+    np.testing.assert_allclose(got[:, 0], mesh.x[pts], rtol=0, atol=1.5e-10)
+    np.testing.assert_allclose(got[:, 1], cp[pts], rtol=0, atol=1.5e-10)
