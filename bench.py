@@ -21,8 +21,10 @@ Measured in one run:
                    bit) on the workload's limit cycle: window means of Cl/Cd,
                    beside FP64 vs FP64 from a 1e-12-perturbed start
   accuracy_converged   FP32 vs FP64 on a converging first-order case
-  batch64          64 cases on 4 uneven meshes in one batch (FP32 Mcsu/s)
-  batch256         256 cases on 256 meshes of 2k-150k cells (~8.8M cells, HBM-resident)
+  batch64          64 cases on 4 uneven meshes in one batch (FP32 Mcsu/s); per-case
+                   |dCl| against FP64 beside FP64 from a 1e-12-perturbed start
+  batch256         256 cases on 256 meshes of 2k-150k cells (~9.5M cells, HBM-resident),
+                   the same accuracy figures
   cpu_baseline     the FP64 CPU oracle on this host's cores (OpenMP build), on a
                    bounded sample of steps of the same mesh; one-thread figure beside it
 """
@@ -168,6 +170,22 @@ def converged_accuracy(mesh, max_steps, epsl=1e-7):
                 rho_rel_l2=float(np.linalg.norm(WF[:, 0] - WA[:, 0]) / np.linalg.norm(WA[:, 0])))
 
 
+def perturbed_fp64_control(meshes, case_mesh, flows, nsteps):
+    """The same batch in FP64 from a start perturbed by 1e-12 part, run as
+    long.  Its per-case Cl against the unperturbed FP64 batch shows how far
+    each case's own trajectory separates over the run (limit cycles and
+    transients amplify any difference), the scale for the FP32 differences."""
+    from gpueularsolver2d_amd import Batch
+    P = Batch(meshes, case_mesh, flows, precision="fp64")
+    rng = np.random.default_rng(0)
+    for i in range(len(case_mesh)):
+        W = P.state(i)
+        P.set_state(i, W * (1.0 + 1e-12 * rng.standard_normal(W.shape)))
+    _, _, st = P.run(nsteps, check=False)
+    cl, cd = P.forces()
+    return cl, cd, st
+
+
 def batch64(steps):
     """64 cases in one batch: four NACA 0012 meshes of uneven size (~3k, 10k,
     50k, 90k triangles, seeds 11-14) x 16 (Mach, alpha) pairs on the grid
@@ -201,6 +219,8 @@ def batch64(steps):
     ok = (stF == 0) & (stA == 0)
     (clF, cdF), (clA, cdA) = F.forces(), A.forces()
     d = np.abs(clF - clA)[ok]
+    clP, _, stP = perturbed_fp64_control(meshes, case_mesh, case_flow, 20 + steps)
+    dP = np.abs(clP - clA)[ok & (stP == 0)]
     return dict(value=round(4.0 * done / t / 1e6, 1), unit="Mcsu/s", cases=64, cells=int(cells),
                 mesh_cells=[m.ncell for m in meshes], steps=steps, launch=F.config(),
                 setup_s=round(tset, 2),
@@ -208,7 +228,8 @@ def batch64(steps):
                                        fp32_step=int(nF[i]), fp64_step=int(nA[i]))
                           for i in range(64) if not ok[i]},
                 dcl_max=float(d.max()), dcl_median=float(np.median(d)),
-                dcd_max=float(np.abs(cdF - cdA)[ok].max()))
+                dcd_max=float(np.abs(cdF - cdA)[ok].max()),
+                fp64_perturbed_dcl_max=float(dP.max()), fp64_perturbed_dcl_median=float(np.median(dP)))
 
 
 def config_c1(steps=2000):
@@ -313,6 +334,8 @@ def batch256(steps, workers=16):
     (clF, cdF), (clA, cdA) = F.forces(), A.forces()
     dcl, dcd = np.abs(clF - clA), np.abs(cdF - cdA)
     worst = int(np.argmax(np.where(ok, dcl, -1.0)))
+    clP, _, stP = perturbed_fp64_control(meshes, list(range(256)), flows, 10 + steps)
+    dP = np.abs(clP - clA)[ok & (stP == 0)]
     achieved = 402.0 * cells / (kt[0] * 1e-3) / 1e9
     return dict(value=round(4.0 * done / t / 1e6, 1), unit="Mcsu/s", cases=256, cells=int(cells),
                 steps=steps, launch=F.config(), stage_tile_ms=kt[0],
@@ -323,6 +346,7 @@ def batch256(steps, workers=16):
                           for i in range(256) if not ok[i]},
                 dcl_max=float(dcl[ok].max()), dcl_median=float(np.median(dcl[ok])),
                 dcd_max=float(dcd[ok].max()),
+                fp64_perturbed_dcl_max=float(dP.max()), fp64_perturbed_dcl_median=float(np.median(dP)),
                 worst_case=dict(case=worst, mesh_cells=meshes[worst].ncell, **flows[worst],
                                 cl_fp64=float(clA[worst])),
                 mesh_generation_s=round(tgen, 1), setup_s=round(tset, 2))

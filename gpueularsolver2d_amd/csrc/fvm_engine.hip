@@ -1354,6 +1354,14 @@ fvm_status fvm_batch_get_state(fvm_batch* b, int32_t icase, double* rho, double*
   });
 }
 
+fvm_status fvm_batch_set_state(fvm_batch* b, int32_t icase, const double* rho, const double* rhou,
+                               const double* rhov, const double* rhoE) {
+  return guard([&] {
+    if (!b) throw Error{FVM_ERR_ARGUMENT, "NULL batch"};
+    b->b->set_state(icase, rho, rhou, rhov, rhoE);
+  });
+}
+
 fvm_status fvm_batch_get_forces(fvm_batch* b, double* cl, double* cd) {
   return guard([&] {
     if (!b) throw Error{FVM_ERR_ARGUMENT, "NULL batch"};

@@ -32,9 +32,9 @@ STATUS = {0: "FVM_OK", 1: "FVM_ERR_ARGUMENT", 2: "FVM_ERR_MESH", 3: "FVM_ERR_ARE
 EXPORTS = ("fvm_default_opts", "fvm_status_string", "fvm_last_error", "fvm_mesh_topology",
            "fvm_create", "fvm_run", "fvm_get_state", "fvm_set_state", "fvm_get_forces",
            "fvm_get_timing", "fvm_destroy", "fvm_batch_create", "fvm_batch_run",
-           "fvm_batch_get_state", "fvm_batch_get_forces", "fvm_batch_get_timing",
-           "fvm_batch_get_config", "fvm_get_config", "fvm_batch_kernel_times",
-           "fvm_kernel_times", "fvm_batch_destroy")
+           "fvm_batch_get_state", "fvm_batch_set_state", "fvm_batch_get_forces",
+           "fvm_batch_get_timing", "fvm_batch_get_config", "fvm_get_config",
+           "fvm_batch_kernel_times", "fvm_kernel_times", "fvm_batch_destroy")
 
 
 class FvmError(RuntimeError):
@@ -96,6 +96,7 @@ def load_library(path=LIB_PATH):
                                    C.POINTER(C.c_void_p)]
     L.fvm_batch_run.argtypes = [C.c_void_p, C.c_int32, C.c_double, _dp, _ip, _ip]
     L.fvm_batch_get_state.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp]
+    L.fvm_batch_set_state.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp]
     L.fvm_batch_get_forces.argtypes = [C.c_void_p, _dp, _dp]
     L.fvm_batch_get_timing.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int64)]
     L.fvm_batch_get_config.argtypes = [C.c_void_p, _ip]
@@ -106,9 +107,9 @@ def load_library(path=LIB_PATH):
     L.fvm_batch_destroy.restype = None
     for name in ("fvm_mesh_topology", "fvm_create", "fvm_run", "fvm_get_state", "fvm_set_state",
                  "fvm_get_forces", "fvm_get_timing", "fvm_batch_create", "fvm_batch_run",
-                 "fvm_batch_get_state", "fvm_batch_get_forces", "fvm_batch_get_timing",
-                 "fvm_batch_get_config", "fvm_get_config", "fvm_batch_kernel_times",
-                 "fvm_kernel_times"):
+                 "fvm_batch_get_state", "fvm_batch_set_state", "fvm_batch_get_forces",
+                 "fvm_batch_get_timing", "fvm_batch_get_config", "fvm_get_config",
+                 "fvm_batch_kernel_times", "fvm_kernel_times"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -288,6 +289,10 @@ class Batch:
         out = [np.zeros(self.ncells[icase]) for _ in range(4)]
         _check(load_library().fvm_batch_get_state(self.h, icase, *(_dptr(a) for a in out)))
         return np.stack(out, axis=1)
+
+    def set_state(self, icase, W):
+        cols = [np.ascontiguousarray(W[:, k], dtype=np.float64) for k in range(4)]
+        _check(load_library().fvm_batch_set_state(self.h, icase, *(_dptr(a) for a in cols)))
 
     def forces(self):
         cl, cd = np.zeros(self.ncase), np.zeros(self.ncase)

@@ -163,6 +163,10 @@ fvm_status fvm_batch_run(fvm_batch *b, int32_t max_steps, double epsl, double *r
                          int32_t *steps_done, int32_t *case_status);
 fvm_status fvm_batch_get_state(fvm_batch *b, int32_t icase, double *rho, double *rhou,
                                double *rhov, double *rhoE);
+/* Replaces case icase's state (checkpoint restore), as fvm_set_state does for
+ * a single solver; its step count is kept. */
+fvm_status fvm_batch_set_state(fvm_batch *b, int32_t icase, const double *rho, const double *rhou,
+                               const double *rhov, const double *rhoE);
 fvm_status fvm_batch_get_forces(fvm_batch *b, double *cl, double *cd); /* [ncase] each */
 fvm_status fvm_batch_get_timing(fvm_batch *b, double *ms, int64_t *cell_stage_updates);
 /* Average device time of one launch of each tiled kernel of an RK stage, from

@@ -143,3 +143,21 @@ def test_oversized_tiles_are_argument_errors(meshes):
             Solver(meshes[1], precision=precision, tile_cells=6000, threads=threads, **FLOWS[0])
         assert e.value.name == "FVM_ERR_ARGUMENT"
         assert "LDS" in str(e.value) or "too large" in str(e.value)
+
+
+def test_batch_set_state_restores_a_case(meshes):
+    # a case restored into a fresh batch continues exactly as in the batch it
+    # came from (the step count is kept: the state alone sets the update)
+    flows = [FLOWS[0], FLOWS[1], FLOWS[2]]
+    for precision in ("fp64", "fp32"):
+        a = Batch(meshes, [0, 1, 0], flows, precision=precision)
+        a.run(5)
+        W = a.state(2)
+        b = Batch(meshes, [0, 1, 0], flows, precision=precision)
+        b.set_state(2, W)
+        assert np.array_equal(b.state(2), W)
+        a.run(15)
+        b.run(10)
+        assert np.array_equal(a.state(2), b.state(2))
+        with pytest.raises(FvmError):
+            b.set_state(3, W)
