@@ -54,8 +54,8 @@ inline int stage_slope_stride(int max_local) {
 template <typename R>
 inline size_t stage_tile_lds(int max_r4_local, int max_ring_local, int max_w_local, int max_local,
                              int max_edges) {
-  // W[MW] | gid[MW rounded up to 4] | union { R[M4] | R1[MR] ; grad + bounds + F + SR }
-  const size_t res = (size_t)(max_r4_local + max_ring_local) * sizeof(SVec4<R>);
+  // W[MW] | gid[MW rounded up to 4] | union { R[M4 + 1] | R1[MR + 1] ; grad + bounds + F + SR }
+  const size_t res = (size_t)(max_r4_local + max_ring_local + 2) * sizeof(SVec4<R>);
   const size_t flux = flux_tile_lds<R>(max_local, max_edges) - (size_t)max_local * sizeof(SVec4<R>);
   return (size_t)max_w_local * sizeof(SVec4<R>) + (size_t)((max_w_local + 3) & ~3) * sizeof(int) +
          (res > flux ? res : flux);

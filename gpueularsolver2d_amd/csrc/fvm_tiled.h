@@ -171,15 +171,6 @@ __device__ __forceinline__ int tile_of_block(int bid, int ntiles) {
   return x * q + (x < r ? x : r) + k;
 }
 
-// The addend of a conditional sum `if (in) q += v`, written without a branch:
-// adding -0 leaves every q unchanged (+0 + -0 = +0, -0 + -0 = -0, and q is
-// never denormal under FP32 flush-to-zero), so `q += skip_add(in, v)` gives
-// the same bits as the branch, in the same order.
-template <typename R>
-__device__ __forceinline__ R skip_add(bool in, R v) {
-  return in ? v : R(-0.0);
-}
-
 template <typename R>
 __device__ __forceinline__ Vec4<R> ld(const SVec4<R>& s) {
   return Vec4<R>{s.d, s.du, s.dv, s.de};
@@ -585,7 +576,7 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
 
 // --------------------------------------------------------------------------
 // One kernel per RK stage (see the header comment).
-// LDS: W[MW] | gid[MW4] | union { R[M4] | R1[MR] ;  grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME] }
+// LDS: W[MW] | gid[MW4] | union { R[M4 + 1] | R1[MR + 1] ;  grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME] }
 // (ML: the slope row stride, S or max_local; MW4: MW rounded up to 4)
 // R holds the previous stage's residual of the tile and rings 1..4, one load
 // per cell, so that smoothing sweep 1 gathers from LDS through the 16-bit
@@ -610,13 +601,13 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   if (a.apply_update ? !cc.active : cc.status != FVM_OK) return;
   const FlowT<R> f = a.flow[d.icase];
   const int nt = d.ncell, NG = nt + d.nhalo, NW = NG + d.n2, NR = NW + d.n3, N4 = NR + d.n4;
-  const int M4 = a.max_r4_local, MW = a.max_w_local, ME = a.max_edges;
+  const int M4 = a.max_r4_local, MR = a.max_ring_local, MW = a.max_w_local, ME = a.max_edges;
   const int ML = S > 0 ? S : a.max_local;
   const int MW4 = (MW + 3) & ~3;
   SVec4<R>* sW = reinterpret_cast<SVec4<R>*>(smem);
   int* sGid = reinterpret_cast<int*>(sW + MW);
   SVec4<R>* sR = reinterpret_cast<SVec4<R>*>(sGid + MW4);
-  SVec4<R>* sR1 = sR + M4;
+  SVec4<R>* sR1 = sR + M4 + 1;
   R* sG = reinterpret_cast<R*>(sR);
   R* sB = sG + 8 * ML;
   SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
@@ -655,6 +646,15 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     // 1. the previous stage's residual of the tile and rings 1..4 (1..2
     //    without smoothing)
     const int n1 = a.smoothing ? N4 : NW;
+    // The smoothing sums `if (neighbour is a cell) q += R[neighbour]` read a
+    // boundary side from a slot past the end of R (and of R1) that holds -0:
+    // adding -0 leaves every q unchanged (+0 + -0 = +0, -0 + -0 = -0, and q
+    // is never denormal under FP32 flush-to-zero), so the sums keep the
+    // reference's bits and order without a select per component.
+    if (threadIdx.x == 0) {
+      sR[M4] = SVec4<R>{R(-0.0), R(-0.0), R(-0.0), R(-0.0)};
+      sR1[MR] = SVec4<R>{R(-0.0), R(-0.0), R(-0.0), R(-0.0)};
+    }
     for (int j = threadIdx.x; j < n1; j += blockDim.x) {
       const int g = gid(j);
       sR[j] = Rin[g];
@@ -674,15 +674,13 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
         const SVec4<R> ri = sR[j];
         SVec4<R> v[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) v[k] = sR[lk[k] >= 0 ? lk[k] : j];
+        for (int k = 0; k < 3; k++) v[k] = sR[lk[k] >= 0 ? lk[k] : M4];
         R q0 = R(0.0), q1 = R(0.0), q2 = R(0.0), q3 = R(0.0);
         int clog = 0;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-          const bool in = lk[k] >= 0;
-          q0 += skip_add(in, v[k].d); q1 += skip_add(in, v[k].du);
-          q2 += skip_add(in, v[k].dv); q3 += skip_add(in, v[k].de);
-          clog += in;
+          q0 += v[k].d; q1 += v[k].du; q2 += v[k].dv; q3 += v[k].de;
+          clog += lk[k] >= 0;
         }
         const R den = R(1.0) + eps * R(clog);
         sR1[j] = SVec4<R>{(q0 * eps + ri.d) / den, (q1 * eps + ri.du) / den,
@@ -711,15 +709,13 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
         const int lk[3] = {ln.x, ln.y, ln.z};
         SVec4<R> v[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) v[k] = sR1[lk[k] >= 0 ? lk[k] : j];
+        for (int k = 0; k < 3; k++) v[k] = sR1[lk[k] >= 0 ? lk[k] : MR];
         R q0 = R(0.0), q1 = R(0.0), q2 = R(0.0), q3 = R(0.0);
         int clog = 0;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-          const bool in = lk[k] >= 0;
-          q0 += skip_add(in, v[k].d); q1 += skip_add(in, v[k].du);
-          q2 += skip_add(in, v[k].dv); q3 += skip_add(in, v[k].de);
-          clog += in;
+          q0 += v[k].d; q1 += v[k].du; q2 += v[k].dv; q3 += v[k].de;
+          clog += lk[k] >= 0;
         }
         const R den = R(1.0) + eps * R(clog);
         const R p0 = (q0 * eps + ri.d) / den, p1 = (q1 * eps + ri.du) / den;
