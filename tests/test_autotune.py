@@ -1,0 +1,39 @@
+"""The launch-shape autotuner: its cache (CPU) and a short tuning run (GPU)."""
+import json
+
+import pytest
+
+from gpueularsolver2d_amd import autotune, naca0012_mesh
+
+
+def test_cache_lookup_and_best_config_hit(tmp_path, monkeypatch):
+    # a cached entry for this device/scheme/size bucket is returned without tuning
+    monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "cache.json"))
+    dev = autotune._device_name()
+    key = autotune._key("fp32", 2, True, 200000, dev)
+    with open(autotune.CACHE, "w") as f:
+        json.dump({key: dict(tile_cells=259, threads=512, ms_per_step=0.05, ncell=198578)}, f)
+    assert autotune.lookup("fp32", 2, True, 200000) == dict(tile_cells=259, threads=512,
+                                                             ms_per_step=0.05, ncell=198578)
+    # sizes in the same half-octave bucket share the entry; other schemes do not
+    assert autotune.lookup("fp32", 2, True, 210000) is not None
+    assert autotune.lookup("fp64", 2, True, 200000) is None
+    assert autotune.lookup("fp32", 1, True, 200000) is None
+    hit_mesh = type("M", (), {"ncell": 205000})()  # only its size is read on a hit
+    assert autotune.best_config(hit_mesh, "fp32", 2, True) == dict(tile_cells=259, threads=512)
+
+
+@pytest.mark.gpu
+def test_tune_times_candidates_and_skips_invalid(tmp_path, monkeypatch):
+    monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "cache.json"))
+    m = naca0012_mesh(6000, seed=12)
+    cands = [(0, 256), (64, 128), (6000, 512)]     # the last tile needs too much LDS
+    r = autotune.tune(m, precision="fp32", candidates=cands, steps=10, write_cache=True)
+    rows = r["table"]
+    assert len(rows) == 3 and "error" in rows[2] and "error" not in rows[0]
+    ok = [row for row in rows if "error" not in row]
+    assert all(row["ms_per_step"] > 0 and row["flux_tile_ms"] > 0 for row in ok)
+    assert r["ms_per_step"] == min(row["ms_per_step"] for row in ok)
+    assert (r["tile_cells"], r["threads"]) in [(row["tile_cells"], row["threads"]) for row in ok]
+    # the result is cached under this device and size bucket
+    assert autotune.lookup("fp32", 2, True, m.ncell)["tile_cells"] == r["tile_cells"]
