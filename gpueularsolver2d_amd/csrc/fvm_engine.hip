@@ -672,7 +672,6 @@ void Batch::setup_plain() {
   int nblk_all = 0;
   for (int c = 0; c < ncase; c++) {
     const Topology& t = topo_[cases_[c].mesh];
-    const size_t C = t.ncell, E = t.nedge, nw = std::max(t.nwall, 1), nf = std::max(t.nfar, 1);
     const int nblk = t.ncell / 64 + 1;  // gpu_solver.cu:2217 block count at 64 threads
     h_ctl_[c].tile0 = nblk_all;
     h_ctl_[c].ntile = nblk;
