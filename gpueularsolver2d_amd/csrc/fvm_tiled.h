@@ -409,16 +409,23 @@ __device__ __forceinline__ R gather_cells(const TiledArgs<R>& a, const TileDesc&
 
 // Fixed-order sum of one double per thread over the workgroup (wave
 // shuffles, then waves in order); the result is valid in thread 0.
-__device__ __forceinline__ double block_sum(double acc) {
+// Its two halves, for a kernel that has a barrier between them anyway: each
+// wave's butterfly sum into wsum[wave], then (after the barrier) the waves'
+// sums in order, valid in thread 0.
+__device__ __forceinline__ void wave_sums(double acc, double* wsum) {
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-  __shared__ double wsum[16];
-  const int wv = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) wsum[wv] = acc;
-  __syncthreads();
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+}
+__device__ __forceinline__ double sum_of_waves(const double* wsum) {
   double s = 0.0;
-  if (threadIdx.x == 0)
-    for (int k = 0; k < (int)(blockDim.x + 63) / 64; k++) s += wsum[k];
+  for (int k = 0; k < (int)(blockDim.x + 63) / 64; k++) s += wsum[k];
   return s;
+}
+__device__ __forceinline__ double block_sum(double acc) {
+  __shared__ double wsum[16];
+  wave_sums(acc, wsum);
+  __syncthreads();
+  return threadIdx.x == 0 ? sum_of_waves(wsum) : 0.0;
 }
 
 // --------------------------------------------------------------------------
@@ -745,7 +752,13 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
       sGid[j] = g;
     }
   }
+  // the tile's FP64 partial of (rho - rho0)^2 is complete once the update
+  // is: its wave sums go before this barrier, the tile's sum after it
+  __shared__ double wpart[16];
+  const bool partial_here = a.close_step && a.apply_update;
+  if (partial_here) wave_sums(acc, wpart);
   __syncthreads();
+  if (partial_here && threadIdx.x == 0) a.partial[t] = sum_of_waves(wpart);
   // 4. gradient and bounds of the tile and ring 1 from the updated state
   if (a.order >= 2) {
     for (int j = threadIdx.x; j < NG; j += blockDim.x) {
@@ -769,10 +782,6 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   __syncthreads();
   // 6. ordered gather: this stage's residual and time step
   gather_cells<R>(a, d, f, sF, sS, a.res, a.dt, sl0, area0);
-  if (a.close_step && a.apply_update) {
-    const double s = block_sum(acc);
-    if (threadIdx.x == 0) a.partial[t] = s;
-  }
   // the previous step's close, by the workgroup of the case's first tile once
   // its own tile is done.  The other tiles of this launch read `active` only
   // (a tile that sees the old value computes a residual that the next launch,
