@@ -604,8 +604,10 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   const TileDesc d = a.tiles[t];
   const CaseCtl& cc = a.ctl[d.icase];
   // the first launch of a run prepares every healthy case; later ones only
-  // the cases still iterating
-  if (a.apply_update ? !cc.active : cc.status != FVM_OK) return;
+  // the cases still iterating, a test made once the previous stage's
+  // residual has been requested (below), so that those loads do not wait
+  // for the control block
+  if (!a.apply_update && cc.status != FVM_OK) return;
   const FlowT<R> f = a.flow[d.icase];
   const int nt = d.ncell, NG = nt + d.nhalo, NW = NG + d.n2, NR = NW + d.n3, N4 = NR + d.n4;
   const int M4 = a.max_r4_local, MR = a.max_ring_local, MW = a.max_w_local, ME = a.max_edges;
@@ -667,6 +669,7 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
       sR[j] = Rin[g];
       if (j < NW) sGid[j] = g;
     }
+    if (!cc.active) return;  // the same for the whole workgroup
     __syncthreads();
     // 2. smoothing sweep 1 on the tile and rings 1..3
     if (a.smoothing) {
