@@ -381,6 +381,8 @@ def main():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--fp32-only", action="store_true",
                     help="time only the FP32 path (for profiler runs)")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="time only the FP64 one-kernel-per-stage path (for profiler runs)")
     ap.add_argument("--batch256-fp32-only", action="store_true",
                     help="only run the FP32 256-case batch for --batch256-steps steps (for "
                          "profiler runs)")
@@ -419,6 +421,15 @@ def main():
     flow = dict(mach=0.8, alpha_deg=1.25, cfl=1.5)
     csu = 4.0 * C * args.steps
 
+    if args.fp64_only:
+        s64 = Solver(mesh, precision="fp64", kernels="tiled", **flow)
+        t64, _ = timed_run(s64, args.warmup, args.steps)
+        ms64_region, _ = s64.timing()
+        kt64 = s64.kernel_times(reps=20)
+        print(json.dumps({"metric": METRIC, "value": round(csu / t64 / 1e6, 1), "unit": "Mcsu/s",
+                          "precision": "fp64", "steps": args.steps,
+                          "kernel_ms": ms64_region / (4 * args.steps), "kernel_ms_isolated": kt64[0]}))
+        return 0
     s32 = Solver(mesh, precision="fp32", kernels="tiled", tile_cells=args.tile_cells,
                  threads=args.threads, **flow)
     t32, _ = timed_run(s32, args.warmup, args.steps)
