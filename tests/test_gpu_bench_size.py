@@ -1,0 +1,88 @@
+"""GPU parity at the benchmark's own shapes.
+
+The single case is the 198,578-cell NACA 0012 mesh (seed 3), M 0.8, alpha
+1.25 deg, second order with the limiter and smoothing, CFL 1.5, run with the
+launch shape the library picks for it (259-cell tiles of 512 threads, 767
+tiles).  The batch holds the four meshes of the 64-case workload (3k, 10k,
+50k and 90k cells, seeds 11-14).  The FP64 oracle runs multi-threaded here;
+its bits equal the one-thread build's.
+"""
+import numpy as np
+import pytest
+
+from gpueularsolver2d_amd import Batch, Solver, naca0012_mesh
+from oracle_ctypes import LIB_OMP_PATH, OracleMesh, OracleSolver
+
+pytestmark = pytest.mark.gpu
+
+FLOW = dict(mach=0.8, alpha_deg=1.25, cfl=1.5)
+
+
+@pytest.fixture(scope="module")
+def bench_mesh():
+    return naca0012_mesh(200000, seed=3)
+
+
+def test_fp64_bitwise_vs_oracle_at_bench_size(bench_mesh):
+    steps = 12
+    print(f"bench-size FP64 vs oracle, {bench_mesh.ncell} cells, {steps} steps", flush=True)
+    om = OracleMesh(bench_mesh.x, bench_mesh.y, bench_mesh.tri, library=LIB_OMP_PATH)
+    orc = OracleSolver(om, 0.8, 1.25, cfl=1.5)
+    hist_o, _ = orc.run(steps)
+    s = Solver(bench_mesh, precision="fp64", **FLOW)
+    cfg = s.config()
+    assert cfg["tiles"] > 256  # more tiles than CUs: the benchmark's launch shape
+    hist, n, st = s.run(steps)
+    assert st == 0 and n == steps
+    assert np.array_equal(s.state(), orc.state())
+    # Resd sums per-tile FP64 partials in a fixed order of its own
+    np.testing.assert_allclose(hist, hist_o, rtol=1e-12, atol=0)
+
+
+def test_fp32_one_and_two_kernel_paths_agree_at_bench_size(bench_mesh):
+    a = Solver(bench_mesh, precision="fp32", kernels="tiled", **FLOW)
+    b = Solver(bench_mesh, precision="fp32", kernels="split", **FLOW)
+    a.run(20)
+    b.run(20)
+    assert np.array_equal(a.state(), b.state())
+
+
+def test_fp32_tracks_fp64_at_bench_size(bench_mesh):
+    # FP32 bulk against the FP64 path (= the oracle bit for bit) over the
+    # first steps of the transient, before the limiter and smoothing dynamics
+    # amplify rounding differences (about 60-fold from step 10 to step 30
+    # here).  Error of each conservative relative to the L2 norm of the whole
+    # state; measured on MI355X: max 1.7e-7 after 10 steps, 1.2e-5 after 30;
+    # dCl 4.6e-8 and 2.1e-5.
+    tol = {10: dict(state=1e-6, force=1e-6), 30: dict(state=1e-4, force=1e-4)}
+    out = {}
+    for prec in ("fp32", "fp64"):
+        s = Solver(bench_mesh, precision=prec, **FLOW)
+        for steps in tol:
+            _, n, st = s.run(steps)
+            assert st == 0 and n == steps
+            out[prec, steps] = (s.state(), s.forces())
+    for steps, t in tol.items():
+        (W32, (cl32, cd32)), (W64, (cl64, cd64)) = out["fp32", steps], out["fp64", steps]
+        scale = np.linalg.norm(W64)
+        err = [float(np.linalg.norm(W32[:, k] - W64[:, k]) / scale) for k in range(4)]
+        print(f"FP32 vs FP64 after {steps} steps: state error / |W| {err}, "
+              f"dCl {cl32 - cl64:.3e}, dCd {cd32 - cd64:.3e}", flush=True)
+        assert max(err) < t["state"], (steps, err)
+        assert abs(cl32 - cl64) < t["force"] and abs(cd32 - cd64) < t["force"], steps
+
+
+def test_batch_of_the_64_case_meshes_matches_oracle():
+    meshes = [naca0012_mesh(n, seed=s) for n, s in ((3000, 11), (10000, 12), (50000, 13),
+                                                      (90000, 14))]
+    flows = [dict(mach=m, alpha_deg=a, cfl=1.5) for m, a in ((0.5, 0.0), (0.6167, 1.0),
+                                                            (0.7333, 2.0), (0.85, 3.0))]
+    steps = 10
+    b = Batch(meshes, [0, 1, 2, 3], flows, precision="fp64")
+    _, n, st = b.run(steps)
+    assert list(n) == [steps] * 4 and list(st) == [0] * 4
+    for i, (m, f) in enumerate(zip(meshes, flows)):
+        om = OracleMesh(m.x, m.y, m.tri, library=LIB_OMP_PATH)
+        orc = OracleSolver(om, f["mach"], f["alpha_deg"], cfl=1.5)
+        orc.run(steps)
+        assert np.array_equal(b.state(i), orc.state()), f"case {i} ({m.ncell} cells)"
