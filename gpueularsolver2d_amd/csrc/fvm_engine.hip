@@ -240,8 +240,11 @@ class Batch {
   CaseCtl* d_ctl_ = nullptr;
   std::vector<CaseCtl> h_ctl_;
   double* d_partial_ = nullptr;
-  double* d_hist_ = nullptr;
-  int hist_stride_ = 1;
+  double* d_hist_ = nullptr;      // residual history, hist_stride_ doubles per case
+  int hist_stride_ = 0;
+  HistDesc* d_hdesc_ = nullptr;   // where the step closes write it (device)
+  HistDesc h_hdesc_{nullptr, 0, 0};
+  std::vector<double*> old_hist_; // replaced history buffers, freed with the batch
   std::vector<int> state_off_;   // per case: offset in the per-cell state arrays (tiled)
   // tiled buffers (type-erased; R = float or double)
   void* d_flow_ = nullptr;
@@ -364,6 +367,7 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
   if (fused_) steps_per_graph_ += steps_per_graph_ & 1;
   HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&h_pin_), 2 * cases_.size() * sizeof(CaseCtl)));
   for (auto& e : ev_ctl_) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HIP_OK(hipMemcpyAsync(d_hdesc_, &h_hdesc_, sizeof(HistDesc), hipMemcpyHostToDevice, stream_));
   HIP_OK(hipStreamSynchronize(stream_));
 }
 
@@ -372,6 +376,7 @@ Batch::~Batch() {
   for (auto g : graph_)
     if (g) (void)hipGraphExecDestroy(g);
   if (d_hist_) (void)hipFree(d_hist_);
+  for (double* p : old_hist_) (void)hipFree(p);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   for (auto e : ev_ctl_)
@@ -565,6 +570,7 @@ void Batch::setup_tiled() {
       d_lnb_ = arena_.take<int2>(nlnb_all);
     }
     d_partial_ = arena_.take<double>(tiles.size());
+    d_hdesc_ = arena_.take<HistDesc>(1);
     d_tile_min_ = arena_.take<R>(tiles.size());
     d_dtg_ = arena_.take<R>(ncase);
     wcell = arena_.take<int>(nwall_all);
@@ -737,6 +743,7 @@ void Batch::setup_plain() {
     d_ctl_ = arena_.take<CaseCtl>(ncase);
     d_flow64_ = arena_.take<FlowT<double>>(ncase);
     d_partial_ = arena_.take<double>(nblk_all);
+    d_hdesc_ = arena_.take<HistDesc>(1);
     d_cl_ = arena_.take<double>(ncase);
     d_cd_ = arena_.take<double>(ncase);
   };
@@ -857,8 +864,7 @@ TiledArgs<R> Batch::tiled_args() {
   a.max_r4_local = max_r4_local_;
   a.max_ring_local = max_ring_local_;
   a.max_w_local = max_w_local_;
-  a.hist = d_hist_;
-  a.hist_stride = hist_stride_;
+  a.hist = d_hdesc_;
   return a;
 }
 
@@ -898,7 +904,7 @@ void Batch::enqueue_fused_steps(int nsteps) {
     }
   }
   if (nsteps > 0) {
-    HIP_OK(launch_finalize<R>(d_ctl_, static_cast<double*>(d_partial_), d_hist_, hist_stride_,
+    HIP_OK(launch_finalize<R>(d_ctl_, static_cast<double*>(d_partial_), d_hdesc_,
                               (int)cases_.size(), stream_));
   }
 }
@@ -930,7 +936,7 @@ void Batch::enqueue_tiled_steps(int nsteps) {
                                        static_cast<R*>(d_dtg_), (int)cases_.size(), stream_));
       HIP_OK(launch_update_tile<R>(a, threads_, lds_b_, stream_));
     }
-    HIP_OK(launch_finalize<R>(d_ctl_, static_cast<double*>(d_partial_), d_hist_, hist_stride_,
+    HIP_OK(launch_finalize<R>(d_ctl_, static_cast<double*>(d_partial_), d_hdesc_,
                               (int)cases_.size(), stream_));
   }
 }
@@ -996,7 +1002,7 @@ void Batch::enqueue_plain_steps(int nsteps) {
       plain_launch_step(plain_[c], d_ctl_, c, d_flow64_, o_.order, o_.smoothing, o_.global_dt,
                         stream_);
     HIP_OK(hipGetLastError());
-    HIP_OK(launch_finalize<double>(d_ctl_, static_cast<double*>(d_partial_), d_hist_, hist_stride_,
+    HIP_OK(launch_finalize<double>(d_ctl_, static_cast<double*>(d_partial_), d_hdesc_,
                                    (int)cases_.size(), stream_));
   }
 }
@@ -1043,13 +1049,22 @@ void Batch::run(int max_steps, double epsl, double* hist, int* steps_done, int* 
   HIP_OK(hipSetDevice(device_));
   if (max_steps < 0) throw Error{FVM_ERR_ARGUMENT, "max_steps must be >= 0"};
   const int ncase = (int)cases_.size();
-  // history buffer: one row of max_steps+1 per case (the graph captures its address)
-  if (max_steps + 1 > hist_stride_ || !d_hist_) {
-    if (d_hist_) HIP_OK(hipFree(d_hist_));
-    d_hist_ = nullptr;
-    hist_stride_ = std::max(max_steps + 1, 2);
-    HIP_OK(hipMalloc(&d_hist_, (size_t)ncase * hist_stride_ * sizeof(double)));
-    drop_graphs();
+  // history buffer: one row per case, at least max_steps+1 long.  A longer
+  // run gets a new buffer (at least twice the old one) and a new descriptor,
+  // which the step closes read, so the captured graphs stay valid.  The old
+  // buffer is kept until the batch is destroyed (no free, and so no device
+  // synchronisation, here).
+  if ((int64_t)max_steps + 1 > hist_stride_) {
+    if (max_steps == INT32_MAX) throw Error{FVM_ERR_ARGUMENT, "max_steps must be < 2^31 - 1"};
+    const int stride = (int)std::max<int64_t>({(int64_t)max_steps + 1, 2,
+                                               std::min<int64_t>(2LL * hist_stride_, INT32_MAX)});
+    double* p = nullptr;
+    HIP_OK(hipMalloc(&p, (size_t)ncase * stride * sizeof(double)));
+    if (d_hist_) old_hist_.push_back(d_hist_);
+    d_hist_ = p;
+    hist_stride_ = stride;
+    h_hdesc_ = HistDesc{d_hist_, hist_stride_, 0};
+    HIP_OK(hipMemcpyAsync(d_hdesc_, &h_hdesc_, sizeof(HistDesc), hipMemcpyHostToDevice, stream_));
   }
   std::vector<int> before(ncase);
   for (int c = 0; c < ncase; c++) {
@@ -1113,13 +1128,15 @@ void Batch::run(int max_steps, double epsl, double* hist, int* steps_done, int* 
   for (int c = 0; c < ncase; c++)
     last_csu += 4LL * ncell(c) * (int64_t)(h_ctl_[c].step - before[c]);
   if (hist) {
-    std::vector<double> h((size_t)ncase * hist_stride_, 0.0);
-    HIP_OK(hipMemcpy(h.data(), d_hist_, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const size_t row_n = (size_t)max_steps + 1;
+    std::vector<double> h((size_t)ncase * row_n, 0.0);
+    HIP_OK(hipMemcpy2D(h.data(), row_n * sizeof(double), d_hist_, (size_t)hist_stride_ * sizeof(double),
+                       row_n * sizeof(double), ncase, hipMemcpyDeviceToHost));
     for (int c = 0; c < ncase; c++) {
-      double* row = hist + (size_t)c * (max_steps + 1);
+      double* row = hist + (size_t)c * row_n;
       row[0] = 100.0;
       for (int s = before[c] + 1; s <= h_ctl_[c].step && s <= max_steps; s++)
-        row[s] = h[(size_t)c * hist_stride_ + s];
+        row[s] = h[(size_t)c * row_n + s];
     }
   }
   for (int c = 0; c < ncase; c++) {

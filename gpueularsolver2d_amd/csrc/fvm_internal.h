@@ -123,4 +123,13 @@ struct CaseCtl {                // per-case run control, device resident
   int pad_;
 };
 
+// Where each step's Resd goes: row icase of a buffer of `stride` doubles per
+// case, device resident.  A step close reads it when it writes, so a longer
+// history (a new buffer) changes this record and not the captured graphs.
+struct HistDesc {
+  double* base;
+  int stride;
+  int pad_;
+};
+
 }  // namespace fvm

@@ -158,8 +158,7 @@ struct TiledArgs {
   int max_r4_local, max_ring_local, max_w_local;
   int parity;                 // step parity within the run: step-start buffer = w0_cur ^ parity
   int close_prev;             // 1: the workgroup of each case's first tile closes the previous step
-  double* hist;               // residual history, hist_stride per case
-  int hist_stride;
+  const HistDesc* hist;       // residual history
 };
 
 __device__ __forceinline__ int tile_of_block(int bid, int ntiles) {
@@ -207,7 +206,7 @@ __device__ __forceinline__ void add_pair(double& s, double& c, double s2, double
 // partials t, t + blockDim, ... (Neumaier), a butterfly over each wave
 // combines lane pairs lower lane first (both lanes of a pair then hold bitwise
 // the same value), then thread 0 adds the waves in order.
-__device__ inline void close_case(CaseCtl& c, const double* partial, double* hist, int hist_stride,
+__device__ inline void close_case(CaseCtl& c, const double* partial, const HistDesc* hist,
                                   int icase) {
   if (!c.active) return;
   const int n = c.ntile;
@@ -246,7 +245,8 @@ __device__ inline void close_case(CaseCtl& c, const double* partial, double* his
   const double num = drou + 1.0E-32;
   const double resd = c.epsm != 0.0 ? sqrt(num / c.epsm) : (isnan(num) ? num : HUGE_VAL);
   c.resd = resd;
-  if (step < hist_stride) hist[(size_t)icase * hist_stride + step] = resd;
+  const HistDesc h = *hist;
+  if (step < h.stride) h.base[(size_t)icase * h.stride + step] = resd;
   // Resd is +inf, not NaN, when step 1 changed nothing (drou = 0, a state
   // that is already steady, e.g. freestream on a mesh without walls); the
   // reference's loop goes on then (inf > EPSL), and so does this one
@@ -265,8 +265,8 @@ __device__ inline void close_case(CaseCtl& c, const double* partial, double* his
 constexpr int kFinalizeThreads = 256;
 template <typename R>
 __global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(CaseCtl* ctl, const double* partial,
-                                                                   double* hist, int hist_stride) {
-  close_case(ctl[blockIdx.x], partial, hist, hist_stride, blockIdx.x);
+                                                                   const HistDesc* hist) {
+  close_case(ctl[blockIdx.x], partial, hist, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -792,7 +792,7 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   // launch's parity, not from the control block.
   if (a.close_prev && t == cc.tile0) {
     __syncthreads();
-    close_case(a.ctl[d.icase], a.partial, a.hist, a.hist_stride, d.icase);
+    close_case(a.ctl[d.icase], a.partial, a.hist, d.icase);
   }
 }
 
