@@ -864,6 +864,7 @@ TiledArgs<R> Batch::tiled_args() {
   a.max_r4_local = max_r4_local_;
   a.max_ring_local = max_ring_local_;
   a.max_w_local = max_w_local_;
+  a.early_exit = cases_.size() > 1 ? 1 : 0;
   a.hist = d_hdesc_;
   return a;
 }

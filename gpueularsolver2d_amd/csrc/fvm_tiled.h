@@ -158,6 +158,7 @@ struct TiledArgs {
   int max_r4_local, max_ring_local, max_w_local;
   int parity;                 // step parity within the run: step-start buffer = w0_cur ^ parity
   int close_prev;             // 1: the workgroup of each case's first tile closes the previous step
+  int early_exit;             // 1: test a case's active flag before any load (several cases)
   const HistDesc* hist;       // residual history
 };
 
@@ -604,10 +605,11 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   const TileDesc d = a.tiles[t];
   const CaseCtl& cc = a.ctl[d.icase];
   // the first launch of a run prepares every healthy case; later ones only
-  // the cases still iterating, a test made once the previous stage's
-  // residual has been requested (below), so that those loads do not wait
-  // for the control block
-  if (!a.apply_update && cc.status != FVM_OK) return;
+  // the cases still iterating.  With one case the test is made once the
+  // previous stage's residual has been requested (below), so that those
+  // loads do not wait for the control block; the host stops launching once
+  // the case stops.  With several, a stopped case's tiles leave at once.
+  if (a.apply_update ? (a.early_exit && !cc.active) : cc.status != FVM_OK) return;
   const FlowT<R> f = a.flow[d.icase];
   const int nt = d.ncell, NG = nt + d.nhalo, NW = NG + d.n2, NR = NW + d.n3, N4 = NR + d.n4;
   const int M4 = a.max_r4_local, MR = a.max_ring_local, MW = a.max_w_local, ME = a.max_edges;
