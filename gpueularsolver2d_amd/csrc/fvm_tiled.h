@@ -52,6 +52,13 @@ struct alignas(16) SVec4 {
   R d, du, dv, de;
 };
 
+// Two values read together from LDS: a gradient component (d/dx, d/dy) or
+// a bound (max, min) of one cell.
+template <typename R>
+struct alignas(2 * sizeof(R)) Pair {
+  R a, b;
+};
+
 // Three signed 16-bit values in 8 bytes (x: bits 0-15 and 16-31, y: bits
 // 0-15): the gather slots and tile-local neighbour ids of a cell, whose
 // ranges the host checks.  Read every launch, so the narrower record is HBM
@@ -275,7 +282,9 @@ __global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(CaseCtl* ctl
 // ---------------------------------------------------------------------------
 
 // Green-Gauss gradient and neighbourhood bounds of one cell into the LDS
-// arrays sG[8][ML], sB[8][ML] (kernel_Gradient, gpu_solver.cu:225-383).
+// arrays sG[4][ML] of (d/dx, d/dy) pairs and sB[4][ML] of (max, min) pairs,
+// one row per conservative (kernel_Gradient, gpu_solver.cu:225-383).  The
+// flux reads each pair with one LDS instruction.
 // nbk: neighbour codes (>= 0 interior, -1 far field, -2 wall); nbv: the
 // states of the interior neighbours (entries of boundary sides are unused).
 // Callers load all three sides at once, a boundary side reading the cell
@@ -306,12 +315,12 @@ __device__ __forceinline__ void gradient_cell(int j, const Vec4<R>& Wi, const Ce
 #pragma unroll
     for (int q = 0; q < 4; q++) gy[q] += R(0.5) * (ui[q] + cc[q]) * cg.vx[k];
   }
+  Pair<R>* sG2 = reinterpret_cast<Pair<R>*>(sG);
+  Pair<R>* sB2 = reinterpret_cast<Pair<R>*>(sB);
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    sG[q * ML + j] = gx[q] * cg.inv_area;
-    sG[(4 + q) * ML + j] = gy[q] * cg.inv_area;
-    sB[q * ML + j] = fmax(ui[q], mx[q]);
-    sB[(4 + q) * ML + j] = fmin(ui[q], mn[q]);
+    sG2[q * ML + j] = Pair<R>{gx[q] * cg.inv_area, gy[q] * cg.inv_area};
+    sB2[q * ML + j] = Pair<R>{fmax(ui[q], mx[q]), fmin(ui[q], mn[q])};
   }
 }
 
@@ -336,18 +345,22 @@ __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc
       Rr = Rraw;
       if (a.order >= 2) {
         const int l = ev.l, r = ev.r;
+        const Pair<R>* sG2 = reinterpret_cast<const Pair<R>*>(sG);
+        const Pair<R>* sB2 = reinterpret_cast<const Pair<R>*>(sB);
         R dl[4], dr[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          dl[q] = sG[q * ML + l] * e.rlx + sG[(4 + q) * ML + l] * e.rly;
-          dr[q] = sG[q * ML + r] * e.rrx + sG[(4 + q) * ML + r] * e.rry;
+          const Pair<R> gl = sG2[q * ML + l], gr = sG2[q * ML + r];
+          dl[q] = gl.a * e.rlx + gl.b * e.rly;
+          dr[q] = gr.a * e.rrx + gr.b * e.rry;
         }
         const R ul[4] = {WL.d, WL.du, WL.dv, WL.de};
         const R ur[4] = {Rraw.d, Rraw.du, Rraw.dv, Rraw.de};
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          dl[q] = limit(dl[q], ul[q], sB[q * ML + l], sB[(4 + q) * ML + l]);
-          dr[q] = limit(dr[q], ur[q], sB[q * ML + r], sB[(4 + q) * ML + r]);
+          const Pair<R> bl = sB2[q * ML + l], br = sB2[q * ML + r];
+          dl[q] = limit(dl[q], ul[q], bl.a, bl.b);
+          dr[q] = limit(dr[q], ur[q], br.a, br.b);
         }
         L = Vec4<R>{ul[0] + dl[0], ul[1] + dl[1], ul[2] + dl[2], ul[3] + dl[3]};
         Rr = Vec4<R>{ur[0] + dr[0], ur[1] + dr[1], ur[2] + dr[2], ur[3] + dr[3]};
@@ -443,8 +456,8 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
   const int nt = d.ncell, nl = d.ncell + d.nhalo;
   const int ML = a.max_local, ME = a.max_edges;
   SVec4<R>* sW = reinterpret_cast<SVec4<R>*>(smem);
-  R* sG = reinterpret_cast<R*>(sW + ML);  // [8][ML]: d/dx 4 comps, d/dy 4 comps
-  R* sB = sG + 8 * ML;                    // [8][ML]: max 4 comps, min 4 comps
+  R* sG = reinterpret_cast<R*>(sW + ML);  // [4][ML] (d/dx, d/dy) pairs
+  R* sB = sG + 8 * ML;                    // [4][ML] (max, min) pairs
   SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
   R* sS = reinterpret_cast<R*>(sF + ME);
   const SVec4<R>* W = a.w_in + d.st0;
