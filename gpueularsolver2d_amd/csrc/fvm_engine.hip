@@ -609,7 +609,9 @@ void Batch::setup_tiled() {
     if (fused_) {
       std::copy(T.rings.begin(), T.rings.end(), hrings.begin() + ring0[m]);
       for (size_t q = 0; q < T.lnb.size() / 3; q++)
-        hlnb[lnb0[m] + q] = pack_tri16(T.lnb[3 * q], T.lnb[3 * q + 1], T.lnb[3 * q + 2]);
+        hlnb[lnb0[m] + q] = pack_tri16(T.lnb[3 * q], T.lnb[3 * q + 1], T.lnb[3 * q + 2],
+                                       (T.lnb[3 * q] >= 0) + (T.lnb[3 * q + 1] >= 0) +
+                                           (T.lnb[3 * q + 2] >= 0));
     }
     for (int i = 0; i < T.ncell; i++) {
       CellGeom<R>& g = hcg[geo0[m] + i];

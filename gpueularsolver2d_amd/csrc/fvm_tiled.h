@@ -59,16 +59,24 @@ struct alignas(2 * sizeof(R)) Pair {
   R a, b;
 };
 
-// Three signed 16-bit values in 8 bytes (x: bits 0-15 and 16-31, y: bits
-// 0-15): the gather slots and tile-local neighbour ids of a cell, whose
-// ranges the host checks.  Read every launch, so the narrower record is HBM
-// bandwidth saved.
-__host__ __device__ inline int2 pack_tri16(int a, int b, int c) {
+// Three signed 16-bit values and a fourth small one in 8 bytes (x: bits
+// 0-15 and 16-31, y: bits 0-15, then w in bits 16-31): the gather slots and
+// tile-local neighbour ids of a cell (w = its count of interior neighbours),
+// whose ranges the host checks.  Read every launch, so the narrower record
+// is HBM bandwidth saved.
+__host__ __device__ inline int2 pack_tri16(int a, int b, int c, int w = 0) {
   const unsigned lo = ((unsigned)a & 0xffffu) | (((unsigned)b & 0xffffu) << 16);
-  return make_int2((int)lo, (int)((unsigned)c & 0xffffu));
+  return make_int2((int)lo, (int)(((unsigned)c & 0xffffu) | ((unsigned)w << 16)));
 }
 __device__ __forceinline__ int4 unpack_tri16(int2 v) {
-  return make_int4((v.x << 16) >> 16, v.x >> 16, (v.y << 16) >> 16, 0);
+  return make_int4((v.x << 16) >> 16, v.x >> 16, (v.y << 16) >> 16, v.y >> 16);
+}
+
+// LDS index of a neighbour entry for the smoothing sums: its own index, or
+// the -0 slot `dummy` for a boundary side (negative codes become large
+// unsigned values, so one unsigned minimum selects).
+__device__ __forceinline__ int side_slot(int lk, int dummy) {
+  return (int)min((unsigned)lk, (unsigned)dummy);
 }
 
 // Per tile edge: unit normal (n = (vy, vx)/|v|, the reference's convention),
@@ -699,15 +707,13 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
         const SVec4<R> ri = sR[j];
         SVec4<R> v[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) v[k] = sR[lk[k] >= 0 ? lk[k] : M4];
+        for (int k = 0; k < 3; k++) v[k] = sR[side_slot(lk[k], M4)];
         R q0 = R(0.0), q1 = R(0.0), q2 = R(0.0), q3 = R(0.0);
-        int clog = 0;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
           q0 += v[k].d; q1 += v[k].du; q2 += v[k].dv; q3 += v[k].de;
-          clog += lk[k] >= 0;
         }
-        const R den = R(1.0) + eps * R(clog);
+        const R den = R(1.0) + eps * R(ln.w);  // ln.w: CLog
         sR1[j] = SVec4<R>{(q0 * eps + ri.d) / den, (q1 * eps + ri.du) / den,
                           (q2 * eps + ri.dv) / den, (q3 * eps + ri.de) / den};
       }
@@ -734,15 +740,13 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
         const int lk[3] = {ln.x, ln.y, ln.z};
         SVec4<R> v[3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) v[k] = sR1[lk[k] >= 0 ? lk[k] : MR];
+        for (int k = 0; k < 3; k++) v[k] = sR1[side_slot(lk[k], MR)];
         R q0 = R(0.0), q1 = R(0.0), q2 = R(0.0), q3 = R(0.0);
-        int clog = 0;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
           q0 += v[k].d; q1 += v[k].du; q2 += v[k].dv; q3 += v[k].de;
-          clog += lk[k] >= 0;
         }
-        const R den = R(1.0) + eps * R(clog);
+        const R den = R(1.0) + eps * R(ln.w);  // ln.w: CLog
         const R p0 = (q0 * eps + ri.d) / den, p1 = (q1 * eps + ri.du) / den;
         const R p2 = (q2 * eps + ri.dv) / den, p3 = (q3 * eps + ri.de) / den;
         if (fabs(ri.d) > fabs(p0)) ri.d = p0;
