@@ -410,11 +410,10 @@ __device__ __forceinline__ R gather_cells(const TiledArgs<R>& a, const TileDesc&
       const int s = slk[k] >> 1;
       const SVec4<R> F = sF[s];
       T += sS[s];
-      if ((slk[k] & 1) == 0) {
-        r0 += F.d; r1 += F.du; r2 += F.dv; r3 += F.de;
-      } else {
-        r0 -= F.d; r1 -= F.du; r2 -= F.dv; r3 -= F.de;
-      }
+      // r += F on the edge's left side, r -= F on its right: sg * F is
+      // exact, so r + sg * F (fused or not) rounds as r + F or r - F does
+      const R sg = (slk[k] & 1) ? R(-1.0) : R(1.0);
+      r0 += sg * F.d; r1 += sg * F.du; r2 += sg * F.dv; r3 += sg * F.de;
     }
     Rout[d.st0 + g] = SVec4<R>{r0, r1, r2, r3};
     const R area = (i == s0) ? area0 : a.cg[d.geo0 + g].area;
