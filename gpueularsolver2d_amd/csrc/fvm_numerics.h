@@ -123,12 +123,13 @@ __device__ __forceinline__ T limit(T delt, T u, T umax, T umin) {
 
 // FP32: the same limited slope without a division.  For delta > 0,
 // delta * min((Umax-U)/delta, 1) = min(Umax-U, delta); for delta < 0 the
-// product with a negative delta turns the min into max(Umin-U, delta).
+// product with a negative delta turns the min into max(Umin-U, delta).  The
+// bounds hold U (Umin <= U <= Umax), so both cases, and delta = 0, are the
+// clamp of delta to [Umin-U, Umax-U]: one median of three.  A zero slope
+// may come out with the other sign, which changes U + slope only for U = -0.
 template <>
 __device__ __forceinline__ float limit<float>(float delt, float u, float umax, float umin) {
-  if (delt > 0.f) return fminf(umax - u, delt);
-  if (delt < 0.f) return fmaxf(umin - u, delt);
-  return delt;
+  return __builtin_amdgcn_fmed3f(delt, umin - u, umax - u);
 }
 
 // kernel_Calc_Function, gpu_solver.cu:552-566 (reciprocal-multiply form).
