@@ -313,6 +313,11 @@ __device__ __forceinline__ void gradient_cell(int j, const Vec4<R>& Wi, const Ce
       if (k == 0) {
         mx[q] = cc[q];
         mn[q] = cc[q];
+      } else if constexpr (sizeof(R) == 4) {
+        // FP32: the same extrema by min / max instructions (they may differ
+        // only in the sign of a zero)
+        mn[q] = fminf(mn[q], cc[q]);
+        mx[q] = fmaxf(mx[q], cc[q]);
       } else {
         if (cc[q] < mn[q]) mn[q] = cc[q];
         if (cc[q] > mx[q]) mx[q] = cc[q];
