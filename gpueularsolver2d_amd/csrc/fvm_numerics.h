@@ -113,12 +113,15 @@ __device__ __forceinline__ Vec4<T> boundary_ghost(int type, const Vec4<T>& wd, T
 // kernel_Limiter, gpu_solver.cu:456-494: per face and component,
 // delta *= min((Umax-U)/delta, 1) for delta > 0, min((Umin-U)/delta, 1) for
 // delta < 0, unchanged for delta == 0.
+// The two cases share one division: its numerator is picked by the sign of
+// delta, and delta = 0 (for which the division's value is unused) keeps
+// fi = 1.  Each case performs the same operations as the reference's
+// branches, so the result is the same bit for bit.
 template <typename T>
 __device__ __forceinline__ T limit(T delt, T u, T umax, T umin) {
-  T fi = T(1.0);
-  if (delt > T(0)) fi = fmin((umax - u) / delt, T(1.0));
-  if (delt < T(0)) fi = fmin((umin - u) / delt, T(1.0));
-  return delt * fi;
+  const T num = delt > T(0) ? umax - u : umin - u;
+  const T q = fmin(num / delt, T(1.0));
+  return delt * (delt != T(0) ? q : T(1.0));
 }
 
 // FP32: the same limited slope without a division.  For delta > 0,
