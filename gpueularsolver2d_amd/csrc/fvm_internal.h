@@ -93,7 +93,7 @@ TiledMesh build_tiled(const Topology& topo, int tile_cells);
 // ---------------------------------------------------------------------------
 template <typename R>
 struct alignas(16) CellGeom {   // per cell: gradient geometry and area
-  R vx[3], vy[3], inv_area, area;
+  R vx[3], vy[3], inv_area, area;  // FP32: vx, vy scaled by 0.5 / area
 };
 
 template <typename R>

@@ -338,16 +338,28 @@ __device__ __forceinline__ void gradient_cell(int j, const Vec4<R>& Wi, const Ce
         if (cc[q] > mx[q]) mx[q] = cc[q];
       }
     }
+    if constexpr (sizeof(R) == 4) {
+      // FP32: the edge vectors come scaled by 0.5 / area (the ghosts above
+      // only use their direction)
 #pragma unroll
-    for (int q = 0; q < 4; q++) gx[q] += R(0.5) * (ui[q] + cc[q]) * cg.vy[k];
+      for (int q = 0; q < 4; q++) gx[q] += (ui[q] + cc[q]) * cg.vy[k];
 #pragma unroll
-    for (int q = 0; q < 4; q++) gy[q] += R(0.5) * (ui[q] + cc[q]) * cg.vx[k];
+      for (int q = 0; q < 4; q++) gy[q] += (ui[q] + cc[q]) * cg.vx[k];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++) gx[q] += R(0.5) * (ui[q] + cc[q]) * cg.vy[k];
+#pragma unroll
+      for (int q = 0; q < 4; q++) gy[q] += R(0.5) * (ui[q] + cc[q]) * cg.vx[k];
+    }
   }
   Pair<R>* sG2 = reinterpret_cast<Pair<R>*>(sG);
   Pair<R>* sB2 = reinterpret_cast<Pair<R>*>(sB);
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    sG2[q * ML + j] = Pair<R>{gx[q] * cg.inv_area, gy[q] * cg.inv_area};
+    if constexpr (sizeof(R) == 4)
+      sG2[q * ML + j] = Pair<R>{gx[q], gy[q]};
+    else
+      sG2[q * ML + j] = Pair<R>{gx[q] * cg.inv_area, gy[q] * cg.inv_area};
     sB2[q * ML + j] = Pair<R>{fmax(ui[q], mx[q]), fmin(ui[q], mn[q])};
   }
 }
