@@ -52,8 +52,8 @@ def test_fp32_tracks_fp64_at_bench_size(bench_mesh):
     # first steps of the transient, before the limiter and smoothing dynamics
     # amplify rounding differences (about 60-fold from step 10 to step 30
     # here).  Error of each conservative relative to the L2 norm of the whole
-    # state; measured on MI355X: max 1.7e-7 after 10 steps, 1.2e-5 after 30;
-    # dCl 4.6e-8 and 2.1e-5.
+    # state; measured on MI355X: max 1.9e-7 after 10 steps, 6.1e-6 after 30;
+    # dCl 3.1e-8 and 1.7e-5.
     tol = {10: dict(state=1e-6, force=1e-6), 30: dict(state=1e-4, force=1e-4)}
     out = {}
     for prec in ("fp32", "fp64"):
