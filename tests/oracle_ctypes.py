@@ -12,10 +12,12 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
+# FVM_ORACLE_LIBRARY selects another build of the oracle (a copy with a
+# planted mistake, to check that the tests catch it)
+LIB_PATH = os.environ.get("FVM_ORACLE_LIBRARY", os.path.join(ORACLE_DIR, "liboracle.so"))
 # the same oracle with its per-cell / per-edge loops on OpenMP threads (CPU
 # baseline on all cores; the same bits as liboracle.so)
-LIB_OMP_PATH = os.path.join(ORACLE_DIR, "liboracle_omp.so")
+LIB_OMP_PATH = os.environ.get("FVM_ORACLE_OMP_LIBRARY", os.path.join(ORACLE_DIR, "liboracle_omp.so"))
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
