@@ -1,35 +1,38 @@
 #!/usr/bin/env python3
 """Throughput of the explicit FVM Euler solver on one MI355X.
 
-Workload (default): the ~200k-triangle NACA 0012 mesh (seed 3), M = 0.8,
-alpha = 1.25 deg, second order with the limiter, residual smoothing, 4-stage
-RK, CFL 1.5.  A step is one time step (four RK stages) of the whole mesh; the
-metric counts cell-stage updates (4 * cells per step).
+Workload (the value): the ~200k-triangle NACA 0012 mesh (seed 3), M 0.8,
+alpha 1.25 deg, second order with the limiter, residual smoothing, 4-stage
+RK, CFL 1.5, FP32 bulk with FP64 geometry and reductions.  A step is one time
+step (four RK stages) of the whole mesh; the metric counts cell-stage updates
+(4 * cells per step), host wall clock over exactly --steps steps of one
+fvm_run call, after --warmup steps.
 
-Measured in one run:
-  value            FP32-bulk tiled path (FP64 geometry and reductions), Mcsu/s,
-                   host wall clock over exactly --steps steps, bracketed by
-                   device synchronisation
+Measured in the same run:
+  roofline         the dominant kernel (stage_tile, one launch per RK stage):
+                   its average launch time over the timed steps (HIP events on
+                   the solver's stream, launch gaps included) against three byte
+                   counts per cell-stage update: the 4-sweep algorithmic model
+                   (402 B, frac / frac_model), the counter-measured HBM-side
+                   traffic of one launch (frac_measured) and the single-pass
+                   compulsory bytes (98 B, frac_compulsory)
   fp64_tiled / fp64_plain   the same workload in FP64 (tiled kernels; the
                    reference's per-phase kernel structure)
-  roofline         dominant kernel (stage_tile, one launch per RK stage):
-                   algorithmic bytes per launch / average launch time, the
-                   device time of the timed steps (HIP events on the solver's
-                   stream) over the stage launches in them
-  config_c1 / c2   the small configurations: 3k cells first order against the
-                   oracle (bitwise), 50k cells 5000 steps in FP64 and FP32
-  accuracy         FP32 vs FP64 (the FP64 path equals the FP64 oracle bit for
-                   bit) on the workload's limit cycle: window means of Cl/Cd,
-                   beside FP64 vs FP64 from a 1e-12-perturbed start
+  config_c1        3k cells, first order: the GPU FP64 run against the oracle
+  config_c2        50k cells, 5000 steps: FP64, FP32 (autotuned launch shape)
+                   and plain FP64 throughput; FP32 accuracy as window means
+  accuracy         FP32 vs FP64 on the workload's limit cycle (window means)
   accuracy_converged   FP32 vs FP64 on a converging first-order case
-  batch64          64 cases on 4 uneven meshes in one batch (FP32 Mcsu/s); per-case
-                   |dCl| against FP64 beside FP64 from a 1e-12-perturbed start
-  batch256         256 cases on 256 meshes of 2k-150k cells (~9.5M cells, HBM-resident),
-                   the same accuracy figures
-  cpu_baseline     the FP64 CPU oracle on this host's cores (OpenMP build), on a
-                   bounded sample of steps of the same mesh; one-thread figure beside it
+  batch64          64 cases on 4 uneven meshes in one batch (FP32)
+  batch256         256 cases on 256 meshes of 2k-150k cells (~9.5M cells,
+                   HBM-resident), autotuned launch shape, roofline fractions
+  cpu_baseline     the FP64 CPU oracle on this host's cores
+Per-case accuracy of the batches and C2: Cl and Cd averaged over a window of
+steps after the timed ones, FP32 against FP64, beside the same statistics for
+FP64 restarted from a state perturbed by 1e-12 (the flows' own sensitivity).
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -42,17 +45,24 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mcell-stage-updates/s & % HBM peak; batched FP32 vs plain FP64 at Cl err<1e-4"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per cell-stage update of each tiled kernel (four
-# neighbour-dependent sweeps, each input read once and each output written
-# once, int32 ids, E/C = 1.5): flux_tile = sweeps 1+2 = 45 w_s + 17.5 w_g + 24,
-# update_tile = sweeps 3+4 = 25 w_s + 1 w_g + 24 (w_s = state word, w_g =
-# stored geometry word).
+DCL_GATE = 1e-4
+
+
+# Bytes per cell-stage update (w_s: state word, w_g: stored geometry word).
+# Algorithmic model: four neighbour-dependent sweeps per stage, each reading
+# its inputs once and writing its outputs once (int32 ids, E/C = 1.5):
+# gradient + flux = 45 w_s + 17.5 w_g + 24, smoothing + update = 25 w_s + w_g + 24.
+def model_bytes(w):
+    return 70 * w + 18.5 * w + 48
+
+
+# Single pass, every array read once and written once: 18.5 w + 24.
+def compulsory_bytes(w):
+    return 18.5 * w + 24
+
+
 def flux_bytes(w):
     return 45 * w + 17.5 * w + 24
-
-
-def update_bytes(w):
-    return 25 * w + 1 * w + 24
 
 
 def progress(msg):
@@ -60,35 +70,78 @@ def progress(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def sync():
-    """Solver.run and Batch.run return after the device work has finished
-    (the library synchronises its stream before returning), so the timed
-    regions need no further synchronisation; kept as the one place to add it."""
-    return None
-
-
 def timed_run(solver, warmup, steps):
+    """Wall clock of exactly `steps` steps in one run, after `warmup` steps.
+    Solver.run returns once the device work is done (the library waits for its
+    stream), so the timer brackets the device work."""
     solver.run(warmup)
-    sync()
     t0 = time.perf_counter()
     hist, n, st = solver.run(warmup + steps)
-    sync()
     t1 = time.perf_counter()
     if n != warmup + steps or st != 0:
         raise RuntimeError(f"run stopped at step {n} with status {st}")
     return t1 - t0, hist
 
 
-def traffic_from_profiles(workload, precision, kernel):
-    """HBM bytes per launch measured with rocprofv3 counters (FETCH_SIZE +
-    WRITE_SIZE, each scaled by the same counter's reading for a copy kernel
-    of known size), recorded in profiles/traffic.json."""
+def kernel_source_sha():
+    """Hash of the sources of the FP32 tiled kernels and their build flags: the
+    measured traffic in profiles/ applies only to the kernel it was measured on."""
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "gpueularsolver2d_amd", "csrc")
+    for f in ("fvm_tiled.h", "fvm_numerics.h", "fvm_internal.h", "fvm_launch.h",
+              "fvm_tiled_inst.inc", "fvm_tiled32.hip"):
+        with open(os.path.join(csrc, f), "rb") as fh:
+            h.update(fh.read())
+    with open(os.path.join(ROOT, "gpueularsolver2d_amd", "build.py"), "rb") as fh:
+        h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(workload, kernel="stage_tile"):
+    """HBM-side bytes per launch measured with rocprofv3 counters (L2 read
+    requests by size, 32/64/128 B, plus WRITE_SIZE) on the current kernel
+    source, from profiles/traffic.json; None (with the reason) if that file
+    holds no entry for this workload or was measured on another source."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, "no profiles/traffic.json"
     with open(path) as f:
         d = json.load(f)
-    return d.get(f"{workload}:{precision}:{kernel}")
+    e = d.get(f"{workload}:fp32:{kernel}")
+    if not isinstance(e, dict):
+        return None, f"no entry for {workload}"
+    if e.get("kernel_source_sha") != kernel_source_sha():
+        return None, "measured on another kernel source"
+    return e, None
+
+
+def roofline(cells, t_kern_ms, workload, w=4):
+    """Fractions of the 8 TB/s HBM peak for the stage kernel at t_kern_ms per
+    launch over `cells` cells: by the algorithmic model (frac_model, which
+    counts intermediate round trips the fused kernel does not make, so it can
+    pass 1), by counter-measured traffic (frac_measured) and by the
+    compulsory bytes (frac_compulsory)."""
+    t = t_kern_ms * 1e-3
+    model = model_bytes(w) * cells / t / 1e9
+    e, why = measured_traffic(workload)
+    out = dict(bound="hbm", kernel="stage_tile", achieved=round(model, 1), peak=HBM_PEAK_GBS,
+               unit="GB/s", frac=round(model / HBM_PEAK_GBS, 4),
+               traffic=None if e is None else int(e["bytes_per_launch"]),
+               kernel_ms=t_kern_ms,
+               bytes_per_csu_model=model_bytes(w), frac_model=round(model / HBM_PEAK_GBS, 4),
+               bytes_per_csu_compulsory=compulsory_bytes(w),
+               frac_compulsory=round(compulsory_bytes(w) * cells / t / 1e9 / HBM_PEAK_GBS, 4),
+               frac_model_note="achieved / frac / frac_model use the 4-sweep algorithmic bytes "
+                               "(402 B/csu); the fused kernel moves about one read of each array, "
+                               "so this is a throughput-equivalent and can exceed 1")
+    if e is not None:
+        out.update(frac_measured=round(e["bytes_per_launch"] / t / 1e9 / HBM_PEAK_GBS, 4),
+                   measured_gbs=round(e["bytes_per_launch"] / t / 1e9, 1),
+                   measured_bytes_per_cell=round(e["bytes_per_launch"] / cells, 1),
+                   traffic_read=int(e["read_bytes"]), traffic_write=int(e["write_bytes"]))
+    else:
+        out.update(frac_measured=None, traffic_note=why)
+    return out
 
 
 def cpu_baseline(mesh, steps, omp_steps):
@@ -114,6 +167,47 @@ def cpu_baseline(mesh, steps, omp_steps):
                                    sample=f"{n1} time steps, gcc -O2, one thread, {s1:.1f} s"))
 
 
+def perturbed(W, rng):
+    return W * (1.0 + 1e-12 * rng.standard_normal(W.shape))
+
+
+def window_compare(runs, start, end, every):
+    """Advances every run (name -> object with run(n) and forces()) from step
+    `start` to `end`, sampling Cl and Cd every `every` steps; returns per-run
+    arrays (samples x cases) and the cases that stopped (diverged) in any run."""
+    cl = {k: [] for k in runs}
+    cd = {k: [] for k in runs}
+    bad = None
+    for n in range(start + every, end + 1, every):
+        for k, r in runs.items():
+            _, steps, st = r.run(n, check=False)
+            c, d = r.forces()
+            cl[k].append(np.atleast_1d(c))
+            cd[k].append(np.atleast_1d(d))
+            stopped = np.atleast_1d(st) != 0
+            bad = stopped if bad is None else (bad | stopped)
+    return ({k: np.array(v) for k, v in cl.items()}, {k: np.array(v) for k, v in cd.items()},
+            bad)
+
+
+def window_stats(cl, cd, ok, ref="fp64", others=("fp32", "fp64_perturbed")):
+    """Per-case differences of the window means of Cl and Cd against `ref`,
+    over the cases that ran through (ok), and of the last sample."""
+    out = {}
+    for k in others:
+        dm = (cl[k].mean(axis=0) - cl[ref].mean(axis=0))[ok]
+        dd = (cd[k].mean(axis=0) - cd[ref].mean(axis=0))[ok]
+        inst = np.abs(cl[k][-1] - cl[ref][-1])[ok]
+        out[k] = dict(cases=int(ok.sum()), within_gate=int(np.sum(np.abs(dm) < DCL_GATE)),
+                      dcl_mean_abs_median=float(np.median(np.abs(dm))),
+                      dcl_mean_abs_p95=float(np.percentile(np.abs(dm), 95)),
+                      dcl_mean_abs_max=float(np.abs(dm).max()),
+                      dcd_mean_abs_max=float(np.abs(dd).max()),
+                      dcl_last_abs_median=float(np.median(inst)), dcl_last_abs_max=float(inst.max()))
+    out["cl_fp64_std_median"] = float(np.median(cl[ref].std(axis=0)[ok]))
+    return out
+
+
 def limit_cycle_accuracy(mesh, flow, t0, t1, every):
     """FP32 against FP64 on the workload itself.  With the limiter the
     second-order transonic solution does not converge: the residual stalls
@@ -125,8 +219,7 @@ def limit_cycle_accuracy(mesh, flow, t0, t1, every):
     from gpueularsolver2d_amd import Solver
     A = Solver(mesh, precision="fp64", **flow)
     B = Solver(mesh, precision="fp64", **flow)
-    W = B.state()
-    B.set_state(W * (1.0 + 1e-12 * np.random.default_rng(0).standard_normal(W.shape)))
+    B.set_state(perturbed(B.state(), np.random.default_rng(0)))
     F = Solver(mesh, precision="fp32", **flow)
     for s in (A, B, F):
         s.run(t0)
@@ -171,66 +264,24 @@ def converged_accuracy(mesh, max_steps, epsl=1e-7):
                 rho_rel_l2=float(np.linalg.norm(WF[:, 0] - WA[:, 0]) / np.linalg.norm(WA[:, 0])))
 
 
-def perturbed_fp64_control(meshes, case_mesh, flows, nsteps):
-    """The same batch in FP64 from a start perturbed by 1e-12 part, run as
-    long.  Its per-case Cl against the unperturbed FP64 batch shows how far
-    each case's own trajectory separates over the run (limit cycles and
-    transients amplify any difference), the scale for the FP32 differences."""
+def batch_accuracy(meshes, case_mesh, flows, F, start, window, every):
+    """The FP32 batch F (at step `start`) against the FP64 batch and the FP64
+    batch from a 1e-12-perturbed start, both run to `start` first, then all
+    three sampled over [start, start + window]."""
     from gpueularsolver2d_amd import Batch
+    A = Batch(meshes, case_mesh, flows, precision="fp64")
     P = Batch(meshes, case_mesh, flows, precision="fp64")
     rng = np.random.default_rng(0)
     for i in range(len(case_mesh)):
-        W = P.state(i)
-        P.set_state(i, W * (1.0 + 1e-12 * rng.standard_normal(W.shape)))
-    _, _, st = P.run(nsteps, check=False)
-    cl, cd = P.forces()
-    return cl, cd, st
-
-
-def batch64(steps):
-    """64 cases in one batch: four NACA 0012 meshes of uneven size (~3k, 10k,
-    50k, 90k triangles, seeds 11-14) x 16 (Mach, alpha) pairs on the grid
-    M in {0.5, 0.6167, 0.7333, 0.85} x alpha in {0, 1, 2, 3} deg; second order
-    with the limiter and smoothing, CFL 1.5.  FP32 throughput over `steps`
-    steps, and the per-case |dCl| against the same batch in FP64 after them
-    (instantaneous values; see the limit-cycle comparison for their meaning).
-    Cases that diverge (non-finite residual, in FP64 as well) stop at that
-    step; only the cell-stage updates done count."""
-    from gpueularsolver2d_amd import Batch, naca0012_mesh
-    meshes = [naca0012_mesh(n, seed=s) for n, s in ((3000, 11), (10000, 12), (50000, 13), (90000, 14))]
-    flows = [dict(mach=float(m), alpha_deg=float(a), cfl=1.5)
-             for m in np.linspace(0.5, 0.85, 4) for a in (0.0, 1.0, 2.0, 3.0)]
-    case_mesh = [i % 4 for i in range(64)]
-    case_flow = [flows[(i // 4) % 16] for i in range(64)]
-    cells = sum(meshes[m].ncell for m in case_mesh)
-    t0 = time.perf_counter()
-    F = Batch(meshes, case_mesh, case_flow, precision="fp32")
-    tset = time.perf_counter() - t0
-    F.run(20)
-    sync()
-    t0 = time.perf_counter()
-    _, nF, stF = F.run(20 + steps, check=False)
-    sync()
-    t = time.perf_counter() - t0
-    A = Batch(meshes, case_mesh, case_flow, precision="fp64")
-    _, nA, stA = A.run(20 + steps, check=False)
-    # a case whose residual turns non-finite stops there; the scheme itself
-    # (FP64, equal to the oracle) diverges for such cases too
-    done = sum((int(nF[i]) - 20) * meshes[m].ncell for i, m in enumerate(case_mesh))
-    ok = (stF == 0) & (stA == 0)
-    (clF, cdF), (clA, cdA) = F.forces(), A.forces()
-    d = np.abs(clF - clA)[ok]
-    clP, _, stP = perturbed_fp64_control(meshes, case_mesh, case_flow, 20 + steps)
-    dP = np.abs(clP - clA)[ok & (stP == 0)]
-    return dict(value=round(4.0 * done / t / 1e6, 1), unit="Mcsu/s", cases=64, cells=int(cells),
-                mesh_cells=[m.ncell for m in meshes], steps=steps, launch=F.config(),
-                setup_s=round(tset, 2),
-                diverged={str(i): dict(mesh_cells=meshes[case_mesh[i]].ncell, **case_flow[i],
-                                       fp32_step=int(nF[i]), fp64_step=int(nA[i]))
-                          for i in range(64) if not ok[i]},
-                dcl_max=float(d.max()), dcl_median=float(np.median(d)),
-                dcd_max=float(np.abs(cdF - cdA)[ok].max()),
-                fp64_perturbed_dcl_max=float(dP.max()), fp64_perturbed_dcl_median=float(np.median(dP)))
+        P.set_state(i, perturbed(P.state(i), rng))
+    A.run(start, check=False)
+    P.run(start, check=False)
+    cl, cd, bad = window_compare(dict(fp32=F, fp64=A, fp64_perturbed=P), start, start + window,
+                                 every)
+    ok = ~bad
+    out = dict(window=[start, start + window], every=every, diverged=[int(i) for i in np.flatnonzero(bad)])
+    out.update(window_stats(cl, cd, ok))
+    return out
 
 
 def config_c1(steps=2000):
@@ -240,13 +291,13 @@ def config_c1(steps=2000):
     cell's state is bitwise equal, the largest history difference, Cl, Cd."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_ctypes import LIB_OMP_PATH, OracleMesh, OracleSolver
-    from gpueularsolver2d_amd import Solver, naca0012_mesh
-    m = naca0012_mesh(3000, seed=1)
-    kw = dict(mach=0.5, alpha_deg=0.0, cfl=1.5, order=1, smoothing=False)
-    s = Solver(m, precision="fp64", **kw)
+    from gpueularsolver2d_amd import Solver, workloads
+    c = workloads.C1
+    m = workloads.config_mesh(c)
+    s = Solver(m, precision="fp64", **c["flow"], **c["scheme"])
     h, n, _ = s.run(steps)
-    o = OracleSolver(OracleMesh(m.x, m.y, m.tri, library=LIB_OMP_PATH), 0.5, 0.0, cfl=1.5, order=1,
-                     smoothing=False)
+    o = OracleSolver(OracleMesh(m.x, m.y, m.tri, library=LIB_OMP_PATH), c["flow"]["mach"],
+                     c["flow"]["alpha_deg"], cfl=c["flow"]["cfl"], **c["scheme"])
     ho, _ = o.run(steps)
     cl, cd = s.forces()
     clo, cdo = o.forces()
@@ -256,101 +307,117 @@ def config_c1(steps=2000):
                 dcl_vs_oracle=cl - clo, dcd_vs_oracle=cd - cdo)
 
 
-def config_c2(steps=5000):
-    """Config C2: the ~50k-triangle NACA 0012 mesh (seed 2), M 0.8,
-    alpha 1.25, second order with the limiter and smoothing, CFL 1.5, 5000
-    steps, in FP64 (equal to the oracle) and in FP32: throughput of both and
-    the FP32 errors after the 5000 steps (density rel. L2, Cl, Cd).  The flow
-    is still in its transient at step 5000 (see the limit-cycle comparison for
-    what instantaneous differences mean on such flows)."""
-    from gpueularsolver2d_amd import Solver, naca0012_mesh
-    m = naca0012_mesh(50000, seed=2)
-    flow = dict(mach=0.8, alpha_deg=1.25, cfl=1.5)
-    out = dict(cells=m.ncell, steps=steps)
+def config_c2(steps=5000, window=500, every=10):
+    """Config C2: the ~50k-triangle NACA 0012 mesh (seed 2), M 0.8, alpha
+    1.25, second order with the limiter and smoothing, CFL 1.5, 5000 steps.
+    FP32 runs the launch shape the autotuner picks for this mesh (its table is
+    reported); FP64 (tiled, = the oracle bit for bit) and the plain FP64
+    kernels run the library's.  Accuracy: Cl and Cd averaged over the `window`
+    steps after the timed ones, FP32 against FP64 beside FP64 from a
+    perturbed start."""
+    from gpueularsolver2d_amd import Solver, autotune, workloads
+    c = workloads.C2
+    m = workloads.config_mesh(c)
+    flow = c["flow"]
+    tuned = autotune.tune(m, precision="fp32", flow=flow, steps=40, write_cache=False)
+    out = dict(cells=m.ncell, steps=steps,
+               autotune=dict(chosen=dict(tile_cells=tuned["tile_cells"], threads=tuned["threads"]),
+                             table=tuned["table"]))
     sol = {}
-    for prec in ("fp64", "fp32"):
-        s = Solver(m, precision=prec, **flow)
-        sync()
+    for name, kw in (("fp64", dict(precision="fp64")),
+                     ("fp32", dict(precision="fp32", tile_cells=tuned["tile_cells"],
+                                   threads=tuned["threads"])),
+                     ("fp32_library_shape", dict(precision="fp32")),
+                     ("fp64_plain", dict(precision="fp64", kernels="plain"))):
+        s = Solver(m, **kw, **flow)
+        n_run = steps if name != "fp64_plain" else steps // 5
+        s.run(10)
         t0 = time.perf_counter()
-        h, n, _ = s.run(steps)
-        sync()
+        h, n, _ = s.run(10 + n_run)
         t = time.perf_counter() - t0
-        out[prec] = dict(value=round(4.0 * m.ncell * steps / t / 1e6, 1), unit="Mcsu/s",
-                         resd=float(h[-1]), launch=s.config())
-        sol[prec] = s
-    (cl, cd), (clF, cdF) = sol["fp64"].forces(), sol["fp32"].forces()
-    WA, WF = sol["fp64"].state(), sol["fp32"].state()
+        out[name] = dict(value=round(4.0 * m.ncell * n_run / t / 1e6, 1), unit="Mcsu/s",
+                         steps=n_run, resd=float(h[-1]), launch=s.config())
+        sol[name] = s
+    A, F = sol["fp64"], sol["fp32"]
+    # accuracy: the FP32 and FP64 runs continue for the window, beside FP64
+    # from a perturbed copy of the FP64 state at the start of the window
+    P = Solver(m, precision="fp64", **flow)
+    P.run(10 + steps)
+    P.set_state(perturbed(A.state(), np.random.default_rng(0)))
+    (cl, cd), (clF, cdF) = A.forces(), F.forces()
+    WA, WF = A.state(), F.state()
     out.update(cl_fp64=cl, cd_fp64=cd, dcl=clF - cl, dcd=cdF - cd,
                rho_rel_l2=float(np.linalg.norm(WF[:, 0] - WA[:, 0]) / np.linalg.norm(WA[:, 0])))
+    s0 = 10 + steps
+    cls, cds, bad = window_compare(dict(fp32=F, fp64=A, fp64_perturbed=P), s0, s0 + window, every)
+    w = dict(window=[s0, s0 + window], every=every)
+    w.update(window_stats(cls, cds, ~bad))
+    out["window_accuracy"] = w
     return out
 
 
-def _mesh_job(args):
-    from gpueularsolver2d_amd import naca0012_mesh
-    n, seed = args
-    return naca0012_mesh(n, seed=seed)
+def batch64(steps, window=500, every=10):
+    """64 cases in one batch: four NACA 0012 meshes of uneven size (~3k, 10k,
+    50k, 90k triangles, seeds 11-14) x 16 (Mach, alpha) pairs on the grid
+    M in {0.5, 0.6167, 0.7333, 0.85} x alpha in {0, 1, 2, 3} deg; second order
+    with the limiter and smoothing, CFL 1.5.  FP32 throughput over `steps`
+    steps, then the per-case window accuracy.  Cases that diverge (non-finite
+    residual, in FP64 as well) stop at that step; only the cell-stage updates
+    done count."""
+    from gpueularsolver2d_amd import Batch, workloads
+    meshes, case_mesh, case_flow = workloads.batch64()
+    cells = sum(meshes[m].ncell for m in case_mesh)
+    t0 = time.perf_counter()
+    F = Batch(meshes, case_mesh, case_flow, precision="fp32")
+    tset = time.perf_counter() - t0
+    F.run(20)
+    t0 = time.perf_counter()
+    _, nF, stF = F.run(20 + steps, check=False)
+    t = time.perf_counter() - t0
+    done = sum((int(nF[i]) - 20) * meshes[m].ncell for i, m in enumerate(case_mesh))
+    acc = batch_accuracy(meshes, case_mesh, case_flow, F, 20 + steps, window, every)
+    return dict(value=round(4.0 * done / t / 1e6, 1), unit="Mcsu/s", cases=64, cells=int(cells),
+                mesh_cells=[m.ncell for m in meshes], steps=steps, launch=F.config(),
+                setup_s=round(tset, 2), fp32_stopped=[int(i) for i in np.flatnonzero(stF != 0)],
+                window_accuracy=acc)
 
 
-def batch256_meshes(workers=16):
-    """The 256-case workload: meshes, flows (see batch256)."""
-    from concurrent.futures import ProcessPoolExecutor
-    rng = np.random.default_rng(256)
-    sizes = np.exp(rng.uniform(np.log(2000), np.log(150000), 256)).astype(int)
-    mach = rng.uniform(0.3, 0.85, 256)
-    alpha = rng.uniform(-2.0, 4.0, 256)
-    with ProcessPoolExecutor(max_workers=min(workers, os.cpu_count() or 1)) as ex:
-        meshes = list(ex.map(_mesh_job, [(int(n), 1000 + i) for i, n in enumerate(sizes)]))
-    flows = [dict(mach=float(mach[i]), alpha_deg=float(alpha[i]), cfl=1.5) for i in range(256)]
-    return meshes, flows
-
-
-def batch256(steps, workers=16):
+def batch256(steps, window=500, every=10, workers=16):
     """256 cases in one batch, every case on its own mesh: sizes log-uniform in
     [2k, 150k] triangles (mesh seed 1000 + i), M uniform in [0.3, 0.85], alpha
-    uniform in [-2, 4] deg (drawn with numpy seed 256), second order with the
-    limiter and smoothing, CFL 1.5, library-chosen launch shape.  FP32
-    throughput over `steps` steps (all state resident in HBM), then the
-    worst per-case |dCl| and |dCd| against the same batch in FP64 after the
-    same steps.  Meshes are generated in a process pool."""
-    from gpueularsolver2d_amd import Batch
+    uniform in [-2, 4] deg (numpy seed 256), second order with the limiter and
+    smoothing, CFL 1.5.  The autotuner times launch shapes on the batch and the
+    fastest runs `steps` timed FP32 steps (all state resident in HBM); the
+    stage kernel's time gives the roofline fractions; then the per-case
+    window accuracy."""
+    from gpueularsolver2d_amd import Batch, autotune, workloads
     t0 = time.perf_counter()
-    meshes, flows = batch256_meshes(workers)
+    meshes, flows = workloads.batch256(workers)
     tgen = time.perf_counter() - t0
     case_mesh = list(range(256))
     cells = sum(m.ncell for m in meshes)
     t0 = time.perf_counter()
-    F = Batch(meshes, case_mesh, flows, precision="fp32")
-    tset = time.perf_counter() - t0
+    tuned = autotune.tune_batch(meshes, case_mesh, flows, precision="fp32", steps=10)
+    ttune = time.perf_counter() - t0
+    F = Batch(meshes, case_mesh, flows, precision="fp32", tile_cells=tuned["tile_cells"],
+              threads=tuned["threads"])
     F.run(10)
-    sync()
     t0 = time.perf_counter()
     _, nF, stF = F.run(10 + steps, check=False)
-    sync()
     t = time.perf_counter() - t0
+    ms_region, _ = F.timing()
     kt = F.kernel_times(reps=10)
-    A = Batch(meshes, case_mesh, flows, precision="fp64")
-    _, nA, stA = A.run(10 + steps, check=False)
     done = sum((int(nF[i]) - 10) * meshes[i].ncell for i in range(256))
-    ok = (stF == 0) & (stA == 0)
-    (clF, cdF), (clA, cdA) = F.forces(), A.forces()
-    dcl, dcd = np.abs(clF - clA), np.abs(cdF - cdA)
-    worst = int(np.argmax(np.where(ok, dcl, -1.0)))
-    clP, _, stP = perturbed_fp64_control(meshes, list(range(256)), flows, 10 + steps)
-    dP = np.abs(clP - clA)[ok & (stP == 0)]
-    achieved = 402.0 * cells / (kt[0] * 1e-3) / 1e9
+    t_kern = ms_region / (4 * steps)
+    roof = roofline(cells, t_kern, f"batch256-{cells}")
+    roof["kernel_ms_isolated"] = kt[0]
+    acc = batch_accuracy(meshes, case_mesh, flows, F, 10 + steps, window, every)
     return dict(value=round(4.0 * done / t / 1e6, 1), unit="Mcsu/s", cases=256, cells=int(cells),
-                steps=steps, launch=F.config(), stage_tile_ms=kt[0],
-                roofline_frac=round(achieved / HBM_PEAK_GBS, 4), achieved_gbs=round(achieved, 1),
-                traffic=traffic_from_profiles(f"batch256-{cells}", "fp32", "stage_tile"),
-                diverged={str(i): dict(fp32_status=int(stF[i]), fp32_step=int(nF[i]),
-                                       fp64_status=int(stA[i]), fp64_step=int(nA[i]))
-                          for i in range(256) if not ok[i]},
-                dcl_max=float(dcl[ok].max()), dcl_median=float(np.median(dcl[ok])),
-                dcd_max=float(dcd[ok].max()),
-                fp64_perturbed_dcl_max=float(dP.max()), fp64_perturbed_dcl_median=float(np.median(dP)),
-                worst_case=dict(case=worst, mesh_cells=meshes[worst].ncell, **flows[worst],
-                                cl_fp64=float(clA[worst])),
-                mesh_generation_s=round(tgen, 1), setup_s=round(tset, 2))
+                steps=steps, launch=F.config(),
+                autotune=dict(chosen=dict(tile_cells=tuned["tile_cells"], threads=tuned["threads"]),
+                              table=tuned["table"], seconds=round(ttune, 1)),
+                roofline=roof, fp32_stopped=[int(i) for i in np.flatnonzero(stF != 0)],
+                window_accuracy=acc, mesh_generation_s=round(tgen, 1))
 
 
 def main():
@@ -358,8 +425,6 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--cells", type=int, default=200000)
-    ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--acc-window", type=lambda v: [int(x) for x in v.split(":")],
                     default=[30000, 70000, 100],
                     help="start:end:every of the averaging window of the limit-cycle comparison "
@@ -372,6 +437,8 @@ def main():
                     help="steps of the 64-case batch (0 skips it)")
     ap.add_argument("--batch256-steps", type=int, default=3000,
                     help="steps of the 256-case batch (0 skips it)")
+    ap.add_argument("--window", type=int, default=500,
+                    help="steps of the per-case accuracy window of C2 and the batches")
     ap.add_argument("--cpu-steps", type=int, default=100,
                     help="steps of the one-thread CPU sample")
     ap.add_argument("--cpu-omp-steps", type=int, default=200,
@@ -384,17 +451,21 @@ def main():
     ap.add_argument("--fp64-only", action="store_true",
                     help="time only the FP64 one-kernel-per-stage path (for profiler runs)")
     ap.add_argument("--batch256-fp32-only", action="store_true",
-                    help="only run the FP32 256-case batch for --batch256-steps steps (for "
-                         "profiler runs)")
+                    help="only run the FP32 256-case batch for --batch256-steps steps at the "
+                         "library's launch shape (for profiler runs)")
     ap.add_argument("--copy-mb", type=int, default=0,
-                    help="only run one device copy kernel of this many MiB and exit (calibrates "
-                         "the FETCH_SIZE / WRITE_SIZE counters)")
+                    help="only run one device copy kernel of this many MiB and exit (checks the "
+                         "traffic counters against a known size)")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the solver has no multi-GPU path "
+              "(independent cases would be split across GPUs by case)", file=sys.stderr)
+        return 2
     if args.batch256_fp32_only:
-        # the FP32 256-case batch alone (for profiler runs)
-        from gpueularsolver2d_amd import Batch
-        meshes, flows = batch256_meshes()
-        b = Batch(meshes, list(range(256)), flows, precision="fp32")
+        from gpueularsolver2d_amd import Batch, workloads
+        meshes, flows = workloads.batch256()
+        b = Batch(meshes, list(range(256)), flows, precision="fp32", tile_cells=args.tile_cells,
+                  threads=args.threads)
         _, n, st = b.run(args.batch256_steps, check=False)
         kt = b.kernel_times(reps=5)
         print(json.dumps({"batch256_steps": args.batch256_steps, "cells": sum(m.ncell for m in meshes),
@@ -409,16 +480,12 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_mib": args.copy_mb}))
         return 0
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: the solver has no multi-GPU path "
-              "(independent cases would be split across GPUs by case)",
-              file=sys.stderr)
-        return 2
 
-    from gpueularsolver2d_amd import Solver, naca0012_mesh
-    mesh = naca0012_mesh(args.cells, seed=args.seed)
+    from gpueularsolver2d_amd import Solver, workloads
+    cfg = workloads.C3
+    mesh = workloads.config_mesh(cfg)
     C = mesh.ncell
-    flow = dict(mach=0.8, alpha_deg=1.25, cfl=1.5)
+    flow = cfg["flow"]
     csu = 4.0 * C * args.steps
 
     if args.fp64_only:
@@ -448,41 +515,28 @@ def main():
     plain_steps = max(10, args.steps // 4)
     sp = Solver(mesh, precision="fp64", kernels="plain", **flow)
     tpl, _ = timed_run(sp, 5, plain_steps)
-
     v64 = csu / t64 / 1e6
     vpl = 4.0 * C * plain_steps / tpl / 1e6
     progress(f"fp64 tiled {v64:.1f}, fp64 plain {vpl:.1f} Mcsu/s")
 
+    # the stage kernel's average launch time over the timed steps: the device
+    # time of the steps (HIP events on the solver's stream) over their launches
+    t_kern = ms_region / (4 * args.steps)
+    roof = roofline(C, t_kern, f"naca0012-{C}")
+    roof.update(kernel_ms_isolated=kt[0],
+                step_gbs_model=round(model_bytes(4) * v32 * 1e6 / 1e9, 1))
+
     c1 = config_c1() if not args.no_configs else None
-    c2 = config_c2() if not args.no_configs else None
+    c2 = config_c2(window=args.window) if not args.no_configs else None
     progress("configs C1, C2 done")
     acc = limit_cycle_accuracy(mesh, flow, *args.acc_window) if args.acc_window[1] > 0 else None
     progress("limit-cycle accuracy done")
     conv = converged_accuracy(mesh, args.conv_steps) if args.conv_steps > 0 else None
     progress("converged-case accuracy done")
-    b64 = batch64(args.batch_steps) if args.batch_steps > 0 else None
+    b64 = batch64(args.batch_steps, window=args.window) if args.batch_steps > 0 else None
     progress("64-case batch done")
-    b256 = batch256(args.batch256_steps) if args.batch256_steps > 0 else None
+    b256 = batch256(args.batch256_steps, window=args.window) if args.batch256_steps > 0 else None
     progress("256-case batch done")
-
-    w = 4  # FP32 state and stored geometry
-    if kt[1] > 0:   # two kernels per stage: flux_tile dominates; its share of a
-        # step cannot be read off the step's events, so its own launches are timed
-        name, bytes_unit, t_kern = "flux_tile", flux_bytes(w), kt[0]
-    else:           # one kernel per stage carries all four sweeps: the timed
-        # steps' device time over their 4 launches per step (the launch gaps and
-        # the one close launch per graph included)
-        name, bytes_unit = "stage_tile", flux_bytes(w) + update_bytes(w)
-        t_kern = ms_region / (4 * args.steps)
-    achieved = bytes_unit * C / (t_kern * 1e-3) / 1e9
-    roof = dict(bound="hbm", kernel=name, achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
-                traffic=traffic_from_profiles(f"naca0012-{C}", "fp32", name),
-                kernel_ms=t_kern, kernel_ms_isolated=kt[0],
-                bytes_per_csu_model=flux_bytes(w) + update_bytes(w),
-                bytes_per_csu_kernel=bytes_unit,
-                step_gbs_model=round((flux_bytes(w) + update_bytes(w)) * v32 * 1e6 / 1e9, 1))
-
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         cpu = cpu_baseline(mesh, args.cpu_steps, args.cpu_omp_steps)
@@ -493,7 +547,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t32 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"naca0012-{C} (~200k-triangle NACA 0012, seed {args.seed}), "
+        "config": {"workload": f"naca0012-{C} (~200k-triangle NACA 0012, seed {cfg['seed']}), "
                                "M 0.8, alpha 1.25, 2nd order + limiter + smoothing, 4-stage RK, CFL 1.5",
                    "cells": C, "precision": "fp32 bulk, fp64 geometry + reductions",
                    "kernels": "tiled", "launch": s32.config()},

@@ -7,18 +7,23 @@ size:
   * how the tiles fill the device's workgroup slots.
 
 The workgroup size sets how many passes each phase needs over the tile.  With tile_cells = 0
-the library sizes the tiles so that they fill whole waves of slots.  tune() times that choice
-at several workgroup sizes, beside explicit (tile_cells, threads) candidates, with short runs
-on the real mesh, and keeps the fastest per time step.  Results are cached in
-autotune_cache.json next to this file.  The cache key is the device name, precision, scheme
-and mesh size bucket.
+the library sizes the tiles so that they fill whole waves of slots.  tune_batch() times that
+choice at several workgroup sizes, beside explicit (tile_cells, threads) candidates, with
+short runs of the real batch (many cases on uneven meshes in every launch), and keeps the
+fastest per time step.  tune() does the same for one mesh and caches the result in
+autotune_cache.json next to this file, keyed by the device name, precision, scheme and mesh
+size bucket.
+
+Two knobs of a colouring design do not exist here: cases per workgroup is one tile of one
+case (a tile's LDS neighbourhood belongs to one mesh; a small case simply has fewer tiles),
+and there is no edge colouring (each edge writes its own LDS slot and each cell gathers its
+three slots in a fixed order, which keeps the FP64 sums in the reference's order).
 """
 from __future__ import annotations
 
 import json
 import math
 import os
-import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "autotune_cache.json")
@@ -53,38 +58,71 @@ def lookup(precision, order, smoothing, ncell):
     return load_cache().get(_key(precision, order, smoothing, ncell, _device_name()))
 
 
-def tune(mesh, precision="fp32", order=2, smoothing=True, candidates=None, steps=60, flow=None,
-         write_cache=True, verbose=False):
-    """Times each candidate launch shape on `mesh`; returns the best as a dict
-    {tile_cells, threads, ms_per_step, table}."""
-    from .solver import FvmError, Solver
-    flow = flow or dict(mach=0.8, alpha_deg=1.25, cfl=1.5)
-    candidates = candidates or DEFAULT_CANDIDATES
+def default_batch_candidates(auto_tile):
+    """Launch shapes to time for a batch: the library's own tile choice at
+    three workgroup sizes, and tiles of 0.8, 0.9, 1.1 and 1.2 times its
+    512-thread choice at 512 threads."""
+    c = [(0, 256), (0, 384), (0, 512)]
+    c += [(max(64, int(round(auto_tile * f))), 512) for f in (0.8, 0.9, 1.1, 1.2)]
+    return c
+
+
+def tune_batch(meshes, case_mesh, flows, precision="fp32", order=2, smoothing=True,
+               candidates=None, steps=20, verbose=False):
+    """Times each candidate (tile_cells, threads) launch shape on a batch of
+    cases (fvm_batch_*: every launch covers every case) and returns the
+    fastest as {tile_cells, threads, ms_per_step, table}.  Each candidate runs
+    10 steps, then `steps` timed steps from the device events around the run
+    (fvm_batch_get_timing); its stage kernel is also timed alone.
+    candidates: None picks default_batch_candidates around the library's
+    choice for 512 threads."""
+    from .solver import Batch, FvmError
     table = []
-    for tile, threads in candidates:
+
+    def measure(tile, threads):
         try:
-            s = Solver(mesh, precision=precision, order=order, smoothing=smoothing, tile_cells=tile,
-                       threads=threads, **flow)
-            s.run(10)
-            t0 = time.perf_counter()
-            s.run(10 + steps)
-            wall = (time.perf_counter() - t0) / steps * 1e3
-            ms, _ = s.timing()
-            kt = s.kernel_times(reps=20)
-            cfg = s.config()
-            row = dict(tile_cells=cfg["tile_cells"], threads=threads, requested_tile_cells=tile,
-                       ms_per_step=ms / steps, wall_ms_per_step=wall, flux_tile_ms=kt[0],
-                       update_tile_ms=kt[1], tiles=cfg["tiles"], blocks_per_cu=cfg["blocks_per_cu"])
-            s.close()
+            b = Batch(meshes, case_mesh, flows, precision=precision, order=order,
+                      smoothing=smoothing, tile_cells=tile, threads=threads)
+            b.run(10, check=False)
+            b.run(10 + steps, check=False)
+            ms, csu = b.timing()
+            kt = b.kernel_times(reps=5)
+            cfg = b.config()
+            row = dict(tile_cells=cfg["tile_cells"], threads=cfg["threads"],
+                       requested_tile_cells=tile, ms_per_step=ms / steps,
+                       mcsu_per_s=round(csu / ms / 1e3, 1) if ms > 0 else None,
+                       kernel_ms=kt[0], kernel2_ms=kt[1], tiles=cfg["tiles"],
+                       blocks_per_cu=cfg["blocks_per_cu"])
+            b.close()
         except FvmError as e:  # e.g. the tile's LDS footprint exceeds the device limit
             row = dict(tile_cells=tile, threads=threads, error=str(e))
         table.append(row)
         if verbose:
             print(row, flush=True)
+        return row
+
+    if candidates is None:
+        first = measure(0, 512)
+        auto = first.get("tile_cells", 256)
+        candidates = [c for c in default_batch_candidates(auto) if c != (0, 512)]
+    for tile, threads in candidates:
+        measure(tile, threads)
     ok = [r for r in table if "error" not in r]
     best = min(ok, key=lambda r: r["ms_per_step"])
-    out = dict(tile_cells=best["tile_cells"], threads=best["threads"],
-               ms_per_step=best["ms_per_step"], ncell=mesh.ncell, table=table)
+    return dict(tile_cells=best["tile_cells"], threads=best["threads"],
+                ms_per_step=best["ms_per_step"], table=table)
+
+
+def tune(mesh, precision="fp32", order=2, smoothing=True, candidates=None, steps=60, flow=None,
+         write_cache=True, verbose=False):
+    """Times each candidate launch shape on one case of `mesh` (tune_batch
+    with one case); returns the best as {tile_cells, threads, ms_per_step,
+    ncell, table} and caches it per device, scheme and size bucket."""
+    flow = flow or dict(mach=0.8, alpha_deg=1.25, cfl=1.5)
+    r = tune_batch([mesh], [0], [flow], precision, order, smoothing,
+                   candidates or DEFAULT_CANDIDATES, steps, verbose)
+    out = dict(tile_cells=r["tile_cells"], threads=r["threads"], ms_per_step=r["ms_per_step"],
+               ncell=mesh.ncell, table=r["table"])
     if write_cache:
         cache = load_cache()
         cache[_key(precision, order, smoothing, mesh.ncell, _device_name())] = {

@@ -36,6 +36,9 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 static const double kR = 287.06;              // gpu_fvm.cu:8
 static const double kPI = 3.14159265358979;   // gpu_fvm.cu:9
 static const double kRK[4] = {0.0833, 0.2069, 0.4265, 1.0};  // gpu_fvm.cu:10
+// Steps of residual history every batch holds from its creation (a longer
+// run gets a longer buffer when it starts).
+static const int kInitialHistory = 4096;
 
 // fn(0) .. fn(n-1) on up to 16 host threads (per-mesh pre-processing of a
 // batch; the items are independent).  An Error thrown by any item is
@@ -204,30 +207,57 @@ class Batch {
   template <typename R>
   TiledArgs<R> tiled_args();
   template <typename R>
-  void enqueue_tiled_steps(int nsteps);
-  template <typename R>
-  void enqueue_fused_steps(int nsteps);
-  template <typename R>
   void fused_init();
   template <typename R>
   void time_tiled(int reps, double ms[2]);
-  void enqueue_plain_steps(int nsteps);
-  void enqueue_steps(int nsteps);
+  // Where a launch sequence's step closes find the tiles' partial sums and
+  // the residual history descriptor.
+  // host_ctl: the pinned slot where the sequence's last step close leaves the
+  // control blocks.
+  struct RunBufs {
+    double* partial;
+    const HistDesc* hdesc;
+    CaseCtl* host_ctl;
+  };
+  RunBufs own_bufs(CaseCtl* host_ctl) const { return RunBufs{d_partial_, d_hdesc_, host_ctl}; }
+  // Pinned control-block slots: 0 and 1 for the two instances of the full
+  // graph, 2 .. 2 + kLadderSlots - 1 for the ladder graphs, then two for
+  // directly launched chunks and one for the start and end of a run.
+  static constexpr int kLadderSlots = 32;
+  static constexpr int kDirectSlot = 2 + kLadderSlots, kIoSlot = kDirectSlot + 2, kPinSlots = kIoSlot + 1;
+  CaseCtl* pin_slot(int k) const { return h_pin_ + (size_t)k * cases_.size(); }
+  template <typename R>
+  void enqueue_tiled_steps(int nsteps, RunBufs rb);
+  template <typename R>
+  void enqueue_fused_steps(int nsteps, RunBufs rb);
+  void enqueue_plain_steps(int nsteps, RunBufs rb);
+  void enqueue_steps(int nsteps, RunBufs rb);
   void build_graph();
   void drop_graphs();
+  void ensure_history(int64_t rows);
+  template <typename Launch>
+  void run_steps(int max_steps, double epsl, Launch launch);
+  void finish_run(int* steps_done, int* case_status);
+  void sync_ctl();
 
   fvm_scheme_opts o_;
   int device_ = 0;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   hipGraphExec_t graph_[2] = {nullptr, nullptr};  // two instances of the same steps, alternated
+  // graphs of 2^k steps below steps_per_graph_ (largest first), for the steps
+  // of a run that do not fill a whole graph
+  std::vector<std::pair<int, hipGraphExec_t>> ladder_;
+  std::vector<int> before_;          // per case: step count at the start of a run
+  int n_partial_ = 0;                // per-tile (or per-block) partial sums of the step close
   std::vector<Topology> topo_;
   std::vector<TiledMesh> tiled_;
   std::vector<CaseInfo> cases_;
   Arena arena_;
   int tile_cells_ = 0, threads_ = 256, steps_per_graph_ = 50, total_tiles_ = 0;
   int cus_ = 0, blocks_per_cu_ = 0;  // CUs of the device; workgroups of the tile kernel per CU
-  CaseCtl* h_pin_ = nullptr;         // pinned copies of the control blocks, two slots
+  CaseCtl* h_pin_ = nullptr;         // pinned copies of the control blocks (kPinSlots slots)
+  bool ctl_on_host_ = false;         // the host's control blocks are newer than the device's
   hipEvent_t ev_ctl_[2] = {nullptr, nullptr};
   size_t lds_a_ = 0, lds_b_ = 0, lds_s_ = 0;
   bool fused_ = false;      // one kernel per RK stage
@@ -240,7 +270,8 @@ class Batch {
   CaseCtl* d_ctl_ = nullptr;
   std::vector<CaseCtl> h_ctl_;
   double* d_partial_ = nullptr;
-  double* d_hist_ = nullptr;      // residual history, hist_stride_ doubles per case
+  double* h_hist_ = nullptr;      // residual history, hist_stride_ doubles per case, in pinned
+                                  // host memory that the step closes write directly
   int hist_stride_ = 0;
   HistDesc* d_hdesc_ = nullptr;   // where the step closes write it (device)
   HistDesc h_hdesc_{nullptr, 0, 0};
@@ -365,9 +396,12 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
   // the one-kernel-per-stage graph alternates the step-start buffers, so it
   // holds an even number of steps
   if (fused_) steps_per_graph_ += steps_per_graph_ & 1;
-  HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&h_pin_), 2 * cases_.size() * sizeof(CaseCtl)));
+  HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&h_pin_), kPinSlots * cases_.size() * sizeof(CaseCtl)));
   for (auto& e : ev_ctl_) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  HIP_OK(hipMemcpyAsync(d_hdesc_, &h_hdesc_, sizeof(HistDesc), hipMemcpyHostToDevice, stream_));
+  before_.assign(cases_.size(), 0);
+  // a residual history long enough for runs of up to kInitialHistory steps,
+  // so that such runs allocate nothing
+  ensure_history((int64_t)kInitialHistory + 1);
   HIP_OK(hipStreamSynchronize(stream_));
 }
 
@@ -375,8 +409,9 @@ Batch::~Batch() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto g : graph_)
     if (g) (void)hipGraphExecDestroy(g);
-  if (d_hist_) (void)hipFree(d_hist_);
-  for (double* p : old_hist_) (void)hipFree(p);
+  for (auto& l : ladder_) (void)hipGraphExecDestroy(l.second);
+  if (h_hist_) (void)hipHostFree(h_hist_);
+  for (double* p : old_hist_) (void)hipHostFree(p);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   for (auto e : ev_ctl_)
@@ -538,6 +573,7 @@ void Batch::setup_tiled() {
     ncell_state += T.ncell;
   }
   total_tiles_ = (int)tiles.size();
+  n_partial_ = total_tiles_;
   FlowT<R>* flow = nullptr;
   CellGeom<R>* cg = nullptr;
   EdgeRec<R>* edges = nullptr;
@@ -692,6 +728,7 @@ void Batch::setup_plain() {
     h_ctl_[c].status = FVM_OK;
     nblk_all += nblk;
   }
+  n_partial_ = nblk_all;
   auto carve = [&]() {
     for (int c = 0; c < ncase; c++) {
       const Topology& t = topo_[cases_[c].mesh];
@@ -703,8 +740,6 @@ void Batch::setup_plain() {
       p.nwall = t.nwall;
       p.nblk64 = t.ncell / 64 + 1;
       for (int q = 0; q < 4; q++) p.w[q] = arena_.take<double>(C);
-      p.P = arena_.take<double>(C);
-      p.A = arena_.take<double>(C);
       for (int q = 0; q < 4; q++) p.w0[q] = arena_.take<double>(C);
       for (int q = 0; q < 4; q++) p.farw[q] = arena_.take<double>(nf);
       for (int q = 0; q < 4; q++) p.wallw[q] = arena_.take<double>(nw);
@@ -888,8 +923,10 @@ TiledArgs<R> Batch::tiled_args() {
 // batch is in buffer w0_cur ^ (s & 1) (w0_cur is not changed on the device),
 // and nsteps follows an even number of steps of the same run.
 template <typename R>
-void Batch::enqueue_fused_steps(int nsteps) {
+void Batch::enqueue_fused_steps(int nsteps, RunBufs rb) {
   TiledArgs<R> a = tiled_args<R>();
+  a.partial = rb.partial;
+  a.hist = rb.hdesc;
   SVec4<R>* Ra = static_cast<SVec4<R>*>(d_res_);
   SVec4<R>* Rb = static_cast<SVec4<R>*>(d_res_b_);
   R* Da = static_cast<R*>(d_dt_);
@@ -909,10 +946,8 @@ void Batch::enqueue_fused_steps(int nsteps) {
       HIP_OK(launch_stage_tile<R>(a, threads_, lds_s_, stream_));
     }
   }
-  if (nsteps > 0) {
-    HIP_OK(launch_finalize<R>(d_ctl_, static_cast<double*>(d_partial_), d_hdesc_,
-                              (int)cases_.size(), stream_));
-  }
+  if (nsteps > 0)
+    HIP_OK(launch_finalize<R>(d_ctl_, rb.partial, rb.hdesc, rb.host_ctl, (int)cases_.size(), stream_));
 }
 
 template <typename R>
@@ -926,8 +961,10 @@ void Batch::fused_init() {
 }
 
 template <typename R>
-void Batch::enqueue_tiled_steps(int nsteps) {
+void Batch::enqueue_tiled_steps(int nsteps, RunBufs rb) {
   TiledArgs<R> a = tiled_args<R>();
+  a.partial = rb.partial;
+  a.hist = rb.hdesc;
   SVec4<R>* W0 = static_cast<SVec4<R>*>(d_w0_);
   SVec4<R>* Wc = static_cast<SVec4<R>*>(d_wc_);
   for (int s = 0; s < nsteps; s++) {
@@ -942,7 +979,7 @@ void Batch::enqueue_tiled_steps(int nsteps) {
                                        static_cast<R*>(d_dtg_), (int)cases_.size(), stream_));
       HIP_OK(launch_update_tile<R>(a, threads_, lds_b_, stream_));
     }
-    HIP_OK(launch_finalize<R>(d_ctl_, static_cast<double*>(d_partial_), d_hdesc_,
+    HIP_OK(launch_finalize<R>(d_ctl_, rb.partial, rb.hdesc, s == nsteps - 1 ? rb.host_ctl : nullptr,
                               (int)cases_.size(), stream_));
   }
 }
@@ -990,6 +1027,7 @@ void Batch::time_tiled(int reps, double ms[2]) {
   }
   HIP_OK(hipMemcpyAsync(d_ctl_, h_ctl_.data(), h_ctl_.size() * sizeof(CaseCtl), hipMemcpyHostToDevice, stream_));
   HIP_OK(hipStreamSynchronize(stream_));
+  ctl_on_host_ = false;
 }
 
 void Batch::kernel_times(int reps, double ms[2]) {
@@ -1002,28 +1040,30 @@ void Batch::kernel_times(int reps, double ms[2]) {
     time_tiled<float>(reps, ms);
 }
 
-void Batch::enqueue_plain_steps(int nsteps) {
+void Batch::enqueue_plain_steps(int nsteps, RunBufs rb) {
   for (int s = 0; s < nsteps; s++) {
-    for (int c = 0; c < (int)cases_.size(); c++)
-      plain_launch_step(plain_[c], d_ctl_, c, d_flow64_, o_.order, o_.smoothing, o_.global_dt,
-                        stream_);
+    for (int c = 0; c < (int)cases_.size(); c++) {
+      PlainDev p = plain_[c];
+      p.partial = rb.partial + h_ctl_[c].tile0;
+      plain_launch_step(p, d_ctl_, c, d_flow64_, o_.order, o_.smoothing, o_.global_dt, stream_);
+    }
     HIP_OK(hipGetLastError());
-    HIP_OK(launch_finalize<double>(d_ctl_, static_cast<double*>(d_partial_), d_hdesc_,
+    HIP_OK(launch_finalize<double>(d_ctl_, rb.partial, rb.hdesc, s == nsteps - 1 ? rb.host_ctl : nullptr,
                                    (int)cases_.size(), stream_));
   }
 }
 
-void Batch::enqueue_steps(int nsteps) {
+void Batch::enqueue_steps(int nsteps, RunBufs rb) {
   if (o_.kernels == FVM_KERNELS_PLAIN)
-    enqueue_plain_steps(nsteps);
+    enqueue_plain_steps(nsteps, rb);
   else if (fused_ && o_.precision == FVM_FP64)
-    enqueue_fused_steps<double>(nsteps);
+    enqueue_fused_steps<double>(nsteps, rb);
   else if (fused_)
-    enqueue_fused_steps<float>(nsteps);
+    enqueue_fused_steps<float>(nsteps, rb);
   else if (o_.precision == FVM_FP64)
-    enqueue_tiled_steps<double>(nsteps);
+    enqueue_tiled_steps<double>(nsteps, rb);
   else
-    enqueue_tiled_steps<float>(nsteps);
+    enqueue_tiled_steps<float>(nsteps, rb);
 }
 
 void Batch::drop_graphs() {
@@ -1032,56 +1072,87 @@ void Batch::drop_graphs() {
       HIP_OK(hipGraphExecDestroy(g));
       g = nullptr;
     }
+  for (auto& l : ladder_) HIP_OK(hipGraphExecDestroy(l.second));
+  ladder_.clear();
 }
 
-// Two executable instances of the same captured steps: a replay can be
-// queued while the previous one (of the other instance) still runs.
+// Two executable instances of the same captured steps (a replay can be queued
+// while the previous one, of the other instance, still runs), and one graph
+// of each power-of-two step count below it.  A run of any length is then
+// whole graphs and at most one directly launched step: every graph holds an
+// even number of steps, as the one-kernel-per-stage path needs (its
+// step-start buffer alternates by the step's parity within the graph).
 void Batch::build_graph() {
   drop_graphs();
-  hipGraph_t g;
-  HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-  try {
-    enqueue_steps(steps_per_graph_);
-  } catch (...) {
-    (void)hipStreamEndCapture(stream_, &g);
-    throw;
-  }
-  HIP_OK(hipStreamEndCapture(stream_, &g));
-  for (auto& e : graph_) HIP_OK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
-  HIP_OK(hipGraphDestroy(g));
+  // each graph leaves its outcome in its own pinned slot
+  auto capture = [&](int nsteps, int slot) {
+    hipGraph_t g;
+    HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_steps(nsteps, own_bufs(pin_slot(slot)));
+    } catch (...) {
+      (void)hipStreamEndCapture(stream_, &g);
+      throw;
+    }
+    HIP_OK(hipStreamEndCapture(stream_, &g));
+    hipGraphExec_t e = nullptr;
+    HIP_OK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+    HIP_OK(hipGraphDestroy(g));
+    // set the graph up on the device now rather than at its first launch
+    HIP_OK(hipGraphUpload(e, stream_));
+    return e;
+  };
+  graph_[0] = capture(steps_per_graph_, 0);
+  graph_[1] = capture(steps_per_graph_, 1);
+  int k = 2;
+  while (k * 2 < steps_per_graph_) k *= 2;
+  for (; k >= 2 && (int)ladder_.size() < kLadderSlots; k /= 2)
+    ladder_.emplace_back(k, capture(k, 2 + (int)ladder_.size()));
 }
 
-void Batch::run(int max_steps, double epsl, double* hist, int* steps_done, int* case_status) {
-  HIP_OK(hipSetDevice(device_));
-  if (max_steps < 0) throw Error{FVM_ERR_ARGUMENT, "max_steps must be >= 0"};
+// Residual history of at least `rows` entries per case, in pinned host
+// memory: each step close writes its Resd there (one 8-byte store per case
+// and step, whose latency the rest of the launch hides), so a run returns its
+// history without a copy.  A longer run gets a new buffer (at least twice the
+// old one) and a new descriptor, which the step closes read, so the captured
+// graphs stay valid.  The old buffer is kept until the batch is destroyed.
+void Batch::ensure_history(int64_t rows) {
+  if (rows <= hist_stride_) return;
+  const int stride =
+      (int)std::max<int64_t>({rows, 2, std::min<int64_t>(2LL * hist_stride_, INT32_MAX)});
+  double* p = nullptr;
+  HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&p), (size_t)cases_.size() * stride * sizeof(double)));
+  if (h_hist_) old_hist_.push_back(h_hist_);
+  h_hist_ = p;
+  hist_stride_ = stride;
+  h_hdesc_ = HistDesc{h_hist_, hist_stride_, 0};
+  HIP_OK(hipMemcpyAsync(d_hdesc_, &h_hdesc_, sizeof(HistDesc), hipMemcpyHostToDevice, stream_));
+  HIP_OK(hipStreamSynchronize(stream_));
+}
+
+// The step loop of a run: every active case advances one step per step
+// launched.  The remaining steps of the least advanced active case are split
+// into whole chunks of steps_per_graph_ steps, then chunks of 2^k steps, then
+// at most one single step; launch(n, i) queues chunk i of n steps and returns
+// the pinned slot where the chunk's last step close leaves the control
+// blocks.  Chunks are queued one ahead: while chunk i runs, the host reads
+// chunk i - 1's slot and stops queueing once no case is active (a chunk
+// queued after the last case stopped only runs launches that return at
+// once).  Consecutive chunks use different slots.  Allocates nothing.
+template <typename Launch>
+void Batch::run_steps(int max_steps, double epsl, Launch launch) {
   const int ncase = (int)cases_.size();
-  // history buffer: one row per case, at least max_steps+1 long.  A longer
-  // run gets a new buffer (at least twice the old one) and a new descriptor,
-  // which the step closes read, so the captured graphs stay valid.  The old
-  // buffer is kept until the batch is destroyed (no free, and so no device
-  // synchronisation, here).
-  if ((int64_t)max_steps + 1 > hist_stride_) {
-    if (max_steps == INT32_MAX) throw Error{FVM_ERR_ARGUMENT, "max_steps must be < 2^31 - 1"};
-    const int stride = (int)std::max<int64_t>({(int64_t)max_steps + 1, 2,
-                                               std::min<int64_t>(2LL * hist_stride_, INT32_MAX)});
-    double* p = nullptr;
-    HIP_OK(hipMalloc(&p, (size_t)ncase * stride * sizeof(double)));
-    if (d_hist_) old_hist_.push_back(d_hist_);
-    d_hist_ = p;
-    hist_stride_ = stride;
-    h_hdesc_ = HistDesc{d_hist_, hist_stride_, 0};
-    HIP_OK(hipMemcpyAsync(d_hdesc_, &h_hdesc_, sizeof(HistDesc), hipMemcpyHostToDevice, stream_));
-  }
-  std::vector<int> before(ncase);
   for (int c = 0; c < ncase; c++) {
     CaseCtl& k = h_ctl_[c];
-    before[c] = k.step;
+    before_[c] = k.step;
     k.max_steps = max_steps;
     k.epsl = epsl;
     k.active = (k.step < max_steps && k.resd > epsl && k.status == FVM_OK) ? 1 : 0;
   }
-  if (!graph_[0]) build_graph();
-  HIP_OK(hipMemcpyAsync(d_ctl_, h_ctl_.data(), ncase * sizeof(CaseCtl), hipMemcpyHostToDevice, stream_));
+  CaseCtl* io = pin_slot(kIoSlot);  // pinned: the copy is asynchronous
+  std::copy(h_ctl_.begin(), h_ctl_.end(), io);
+  HIP_OK(hipMemcpyAsync(d_ctl_, io, ncase * sizeof(CaseCtl), hipMemcpyHostToDevice, stream_));
+  ctl_on_host_ = false;
   if (fused_ && !r0_valid_) {
     if (o_.precision == FVM_FP64)
       fused_init<double>();
@@ -1090,12 +1161,6 @@ void Batch::run(int max_steps, double epsl, double* hist, int* steps_done, int* 
     r0_valid_ = true;
   }
   HIP_OK(hipEventRecord(ev0_, stream_));
-  // Every active case advances one step per step launched.  Graph replays
-  // are queued one ahead, alternating the two instances: while replay i runs,
-  // the host reads the control blocks copied after replay i - 1 and stops
-  // queueing once no case is active (a replay queued after the last case
-  // stopped only runs launches that return at once).  The steps left over
-  // run without a graph.
   auto min_active_step = [&](const CaseCtl* c) {
     int m = -1;
     for (int i = 0; i < ncase; i++)
@@ -1103,57 +1168,100 @@ void Batch::run(int max_steps, double epsl, double* hist, int* steps_done, int* 
     return m;
   };
   const int m0 = min_active_step(h_ctl_.data());
+  const CaseCtl* last = nullptr;  // the slot of the last chunk launched
   if (m0 >= 0) {
     const int remaining = max_steps - m0;
-    const int nfull = remaining / steps_per_graph_, rest = remaining % steps_per_graph_;
+    const int nfull = remaining / steps_per_graph_;
+    int rest = remaining % steps_per_graph_;
+    int k = 1;  // the largest power of two <= rest
+    while (k * 2 <= rest) k *= 2;
     bool any = true;
-    for (int i = 0; i < nfull && any; i++) {
-      HIP_OK(hipGraphLaunch(graph_[i & 1], stream_));
-      CaseCtl* slot = h_pin_ + (size_t)(i & 1) * ncase;
-      HIP_OK(hipMemcpyAsync(slot, d_ctl_, ncase * sizeof(CaseCtl), hipMemcpyDeviceToHost, stream_));
+    CaseCtl* prev = nullptr;
+    for (int i = 0; any; i++) {
+      int n;
+      if (i < nfull) {
+        n = steps_per_graph_;
+      } else {
+        while (k > 1 && (rest & k) == 0) k /= 2;
+        if (rest == 0) break;
+        n = k > 1 ? k : 1;
+        rest -= n;
+      }
+      CaseCtl* slot = launch(n, i);
       HIP_OK(hipEventRecord(ev_ctl_[i & 1], stream_));
       if (i > 0) {
         HIP_OK(hipEventSynchronize(ev_ctl_[(i - 1) & 1]));
-        any = min_active_step(h_pin_ + (size_t)((i - 1) & 1) * ncase) >= 0;
+        any = min_active_step(prev) >= 0;
       }
+      prev = slot;
     }
-    HIP_OK(hipMemcpyAsync(h_ctl_.data(), d_ctl_, ncase * sizeof(CaseCtl), hipMemcpyDeviceToHost, stream_));
-    HIP_OK(hipStreamSynchronize(stream_));
-    if (rest > 0 && min_active_step(h_ctl_.data()) >= 0) {
-      enqueue_steps(rest);
-      HIP_OK(hipMemcpyAsync(h_ctl_.data(), d_ctl_, ncase * sizeof(CaseCtl), hipMemcpyDeviceToHost, stream_));
-      HIP_OK(hipStreamSynchronize(stream_));
-    }
+    last = prev;
   }
   HIP_OK(hipEventRecord(ev1_, stream_));
   HIP_OK(hipEventSynchronize(ev1_));
+  if (last) std::copy(last, last + ncase, h_ctl_.begin());
   float ms = 0.f;
   HIP_OK(hipEventElapsedTime(&ms, ev0_, ev1_));
   last_ms = ms;
   last_csu = 0;
   for (int c = 0; c < ncase; c++)
-    last_csu += 4LL * ncell(c) * (int64_t)(h_ctl_[c].step - before[c]);
+    last_csu += 4LL * ncell(c) * (int64_t)(h_ctl_[c].step - before_[c]);
+}
+
+void Batch::run(int max_steps, double epsl, double* hist, int* steps_done, int* case_status) {
+  HIP_OK(hipSetDevice(device_));
+  if (max_steps < 0) throw Error{FVM_ERR_ARGUMENT, "max_steps must be >= 0"};
+  if (max_steps == INT32_MAX) throw Error{FVM_ERR_ARGUMENT, "max_steps must be < 2^31 - 1"};
+  const int ncase = (int)cases_.size();
+  ensure_history((int64_t)max_steps + 1);
+  if (!graph_[0]) build_graph();
+  run_steps(max_steps, epsl, [&](int n, int i) {
+    if (n == steps_per_graph_) {
+      HIP_OK(hipGraphLaunch(graph_[i & 1], stream_));
+      return pin_slot(i & 1);
+    }
+    for (size_t k = 0; k < ladder_.size(); k++)
+      if (ladder_[k].first == n) {
+        HIP_OK(hipGraphLaunch(ladder_[k].second, stream_));
+        return pin_slot(2 + (int)k);
+      }
+    CaseCtl* slot = pin_slot(kDirectSlot + (i & 1));
+    enqueue_steps(n, own_bufs(slot));
+    return slot;
+  });
   if (hist) {
+    // the steps this run closed, written by the device into h_hist_ (the
+    // run's end synchronised with the device)
     const size_t row_n = (size_t)max_steps + 1;
-    std::vector<double> h((size_t)ncase * row_n, 0.0);
-    HIP_OK(hipMemcpy2D(h.data(), row_n * sizeof(double), d_hist_, (size_t)hist_stride_ * sizeof(double),
-                       row_n * sizeof(double), ncase, hipMemcpyDeviceToHost));
     for (int c = 0; c < ncase; c++) {
       double* row = hist + (size_t)c * row_n;
       row[0] = 100.0;
-      for (int s = before[c] + 1; s <= h_ctl_[c].step && s <= max_steps; s++)
-        row[s] = h[(size_t)c * row_n + s];
+      for (int s = before_[c] + 1; s <= h_ctl_[c].step && s <= max_steps; s++)
+        row[s] = h_hist_[(size_t)c * hist_stride_ + s];
     }
   }
-  for (int c = 0; c < ncase; c++) {
+  finish_run(steps_done, case_status);
+}
+
+// Per-case results of a run, and the step-start buffer each case now reads:
+// every step of the run alternated it.  The device's copy of the control
+// blocks is brought up to date before the next launch that reads it.
+void Batch::finish_run(int* steps_done, int* case_status) {
+  for (int c = 0; c < (int)cases_.size(); c++) {
     if (steps_done) steps_done[c] = h_ctl_[c].step;
     if (case_status) case_status[c] = h_ctl_[c].status;
+    if (fused_) h_ctl_[c].w0_cur ^= (h_ctl_[c].step - before_[c]) & 1;
   }
-  if (fused_) {
-    // each step of this run alternated the case's step-start buffer
-    for (int c = 0; c < ncase; c++) h_ctl_[c].w0_cur ^= (h_ctl_[c].step - before[c]) & 1;
-    HIP_OK(hipMemcpy(d_ctl_, h_ctl_.data(), ncase * sizeof(CaseCtl), hipMemcpyHostToDevice));
-  }
+  ctl_on_host_ = true;
+}
+
+void Batch::sync_ctl() {
+  if (!ctl_on_host_) return;
+  CaseCtl* io = pin_slot(kIoSlot);
+  std::copy(h_ctl_.begin(), h_ctl_.end(), io);
+  HIP_OK(hipMemcpyAsync(d_ctl_, io, cases_.size() * sizeof(CaseCtl), hipMemcpyHostToDevice, stream_));
+  HIP_OK(hipStreamSynchronize(stream_));
+  ctl_on_host_ = false;
 }
 
 void Batch::get_state(int icase, double* rho, double* rhou, double* rhov, double* rhoe) {
@@ -1221,6 +1329,7 @@ void Batch::set_state(int icase, const double* rho, const double* rhou, const do
 
 void Batch::forces(double* cl, double* cd) {
   HIP_OK(hipSetDevice(device_));
+  sync_ctl();  // forces_tiled reads each case's current step-start buffer
   const int ncase = (int)cases_.size();
   if (o_.kernels == FVM_KERNELS_PLAIN) {
     for (int c = 0; c < ncase; c++)

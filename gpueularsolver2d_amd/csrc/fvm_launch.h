@@ -16,8 +16,8 @@ template <typename R>
 hipError_t launch_stage_tile(const TiledArgs<R>& a, int threads, size_t lds, hipStream_t s);
 // finalize_kernel over ncase control blocks, built with R's options.
 template <typename R>
-hipError_t launch_finalize(CaseCtl* ctl, const double* partial, const HistDesc* hist, int ncase,
-                           hipStream_t s);
+hipError_t launch_finalize(CaseCtl* ctl, const double* partial, const HistDesc* hist,
+                           CaseCtl* host_out, int ncase, hipStream_t s);
 template <typename R>
 hipError_t launch_global_dt_min(const CaseCtl* ctl, const R* tile_min, R* dt_global, int ncase,
                                 hipStream_t s);

@@ -24,6 +24,22 @@ struct FlowT {
   T gamma, R, ui, vi, ain, tin, vin, pis, cfl, eps;
 };
 
+// Minimum over the workgroup (blockDim a multiple of 64, at most 1024): a
+// butterfly of wave shuffles, then thread 0 takes the waves' minima.  The
+// minimum of a set does not depend on the order it is taken in (the values
+// are positive time steps; fmin skips a NaN whatever its position), so the
+// result is the same bits as a sequential scan.  Valid in thread 0.
+template <typename R>
+__device__ __forceinline__ R block_min(R v) {
+  __shared__ R wmin[16];
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) v = fmin(v, wmin[k]);
+  return v;
+}
+
 // kernel_CalculationPTA, gpu_solver.cu:49-60: pressure and sound speed.
 template <typename T>
 __device__ __forceinline__ void pressure_sound(const Vec4<T>& w, T gamma, T& P, T& A) {

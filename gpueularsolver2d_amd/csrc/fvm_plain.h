@@ -14,8 +14,9 @@ namespace fvm {
 
 struct PlainDev {
   int C = 0, E = 0, nfar = 0, nwall = 0, nblk64 = 0;
-  // state (reference W_S: density, density_U, density_V, density_E, P, A)
-  double *w[4] = {}, *P = nullptr, *A = nullptr;
+  // state: the conservatives of the reference's W_S (P, T and A, which it also
+  // stores, are recomputed where they are needed)
+  double *w[4] = {};
   double *w0[4] = {};
   double *farw[4] = {}, *wallw[4] = {};
   // cells

@@ -182,13 +182,12 @@ __global__ void k_dtmin_blocks(PlainDev p, const CaseCtl* ctl, const FlowT<doubl
   if (threadIdx.x == 0) p.blockmin[blockIdx.x] = s[0];
 }
 
-__global__ void k_dtmin_final(PlainDev p, const CaseCtl* ctl, int nblocks) {
+__global__ void __launch_bounds__(kThreads) k_dtmin_final(PlainDev p, const CaseCtl* ctl, int nblocks) {
   if (!ctl->active) return;
-  if (threadIdx.x == 0) {
-    double m = 9999999.0;
-    for (int b = 0; b < nblocks; b++) m = fmin(m, p.blockmin[b]);
-    *p.dtg = m;
-  }
+  double m = 9999999.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) m = fmin(m, p.blockmin[b]);
+  m = block_min(m);
+  if (threadIdx.x == 0) *p.dtg = m;
 }
 
 // kernel_Smooth1, gpu_solver.cu:1635-1673
@@ -222,24 +221,20 @@ __global__ void k_smooth2(PlainDev p, const CaseCtl* ctl, const FlowT<double>* f
   }
 }
 
-// kernel_RKStep, gpu_solver.cu:938-966 (the stored P and A feed the far-field
-// ghosts, as in the reference)
-__global__ void k_rkstep(PlainDev p, const CaseCtl* ctl, const FlowT<double>* flow, double RKc,
-                         int global_dt) {
+// kernel_RKStep, gpu_solver.cu:938-966.  The reference also stores P, T and A
+// of the new state here, and its far-field ghost reads P and A back
+// (gpu_solver.cu:96-98); far_ghost recomputes them from the cell's
+// conservatives with the same formula (kernel_CalculationPTA), which gives
+// the same bits, so this path stores only the conservatives.
+__global__ void k_rkstep(PlainDev p, const CaseCtl* ctl, double RKc, int global_dt) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (!ctl->active || c >= p.C) return;
   double RK = RKc;
   RK *= global_dt ? *p.dtg : p.dt[c];
-  Vec4<double> ww;
-  ww.d = p.w0[0][c] - RK * p.res[0][c] / p.area[c];
-  ww.du = p.w0[1][c] - RK * p.res[1][c] / p.area[c];
-  ww.dv = p.w0[2][c] - RK * p.res[2][c] / p.area[c];
-  ww.de = p.w0[3][c] - RK * p.res[3][c] / p.area[c];
-  double P, A;
-  pressure_sound(ww, flow->gamma, P, A);
-  p.w[0][c] = ww.d; p.w[1][c] = ww.du; p.w[2][c] = ww.dv; p.w[3][c] = ww.de;
-  p.P[c] = P;
-  p.A[c] = A;
+  p.w[0][c] = p.w0[0][c] - RK * p.res[0][c] / p.area[c];
+  p.w[1][c] = p.w0[1][c] - RK * p.res[1][c] / p.area[c];
+  p.w[2][c] = p.w0[2][c] - RK * p.res[2][c] / p.area[c];
+  p.w[3][c] = p.w0[3][c] - RK * p.res[3][c] / p.area[c];
 }
 
 // kernel_drou, gpu_solver.cu:968-1020: 64-cell blocks over the reference's
@@ -285,13 +280,13 @@ void plain_launch_step(const PlainDev& p, CaseCtl* ctl_all, int icase, const Flo
     hipLaunchKernelGGL(k_e2c, dim3(gc), dim3(kThreads), 0, s, p, ctl, flow);
     if (global_dt) {
       hipLaunchKernelGGL(k_dtmin_blocks, dim3(gc), dim3(kThreads), 0, s, p, ctl, flow);
-      hipLaunchKernelGGL(k_dtmin_final, dim3(1), dim3(64), 0, s, p, ctl, gc);
+      hipLaunchKernelGGL(k_dtmin_final, dim3(1), dim3(kThreads), 0, s, p, ctl, gc);
     }
     if (smoothing) {
       hipLaunchKernelGGL(k_smooth1, dim3(gc), dim3(kThreads), 0, s, p, ctl, flow);
       hipLaunchKernelGGL(k_smooth2, dim3(gc), dim3(kThreads), 0, s, p, ctl, flow);
     }
-    hipLaunchKernelGGL(k_rkstep, dim3(gc), dim3(kThreads), 0, s, p, ctl, flow, RK[ir], global_dt);
+    hipLaunchKernelGGL(k_rkstep, dim3(gc), dim3(kThreads), 0, s, p, ctl, RK[ir], global_dt);
   }
   hipLaunchKernelGGL(k_drou, dim3(p.nblk64), dim3(64), 0, s, p, ctl);
 }

@@ -59,6 +59,12 @@ struct alignas(2 * sizeof(R)) Pair {
   R a, b;
 };
 
+// A cell's (1 / area, area), adjacent in its CellGeom record: one load.
+template <typename R>
+__device__ __forceinline__ Pair<R> area_pair(const CellGeom<R>* cg, int i) {
+  return *reinterpret_cast<const Pair<R>*>(&cg[i].inv_area);
+}
+
 // Three signed 16-bit values and a fourth small one in 8 bytes (x: bits
 // 0-15 and 16-31, y: bits 0-15, then w in bits 16-31): the gather slots and
 // tile-local neighbour ids of a cell (w = its count of interior neighbours),
@@ -151,7 +157,8 @@ struct TiledArgs {
   const EdgeRec<R>* edges;
   const SVec4<R>* w_in;       // state read by kernel A
   SVec4<R>* res;              // residual written by kernel A / stage_tile
-  R* dt;                      // local dt written by kernel A / stage_tile (A*CFL/T in global mode)
+  R* dt;                      // local dt written by kernel A / stage_tile (FP32: dt / A; A*CFL/T in
+                              // global mode)
   R* tile_min;                // per tile minimum of A*CFL/T (global dt mode)
   const R* dt_global;         // per case (global dt mode)
   const SVec4<R>* w0;         // step-start state (two-kernel path)
@@ -277,12 +284,21 @@ __device__ inline void close_case(CaseCtl& c, const double* partial, const HistD
 // device-scope fence, which on MI355X would write back a whole L2 per
 // workgroup.  It is instantiated in each precision's translation unit (R is
 // otherwise unused) so that every close of a run, in a stage kernel or here,
-// is the same code built with the same floating-point options.
+// is the same code built with the same floating-point options.  host_out
+// (pinned host memory, may be null) receives each case's control block once
+// it is closed, so the host reads a launch sequence's outcome without a copy
+// of its own.
 constexpr int kFinalizeThreads = 256;
 template <typename R>
 __global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(CaseCtl* ctl, const double* partial,
-                                                                   const HistDesc* hist) {
+                                                                   const HistDesc* hist,
+                                                                   CaseCtl* host_out) {
   close_case(ctl[blockIdx.x], partial, hist, blockIdx.x);
+  // close_case's last writes are thread 0's own
+  if (host_out && threadIdx.x == 0) {
+    host_out[blockIdx.x] = ctl[blockIdx.x];
+    __threadfence_system();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -424,11 +440,15 @@ __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc
 
 // Ordered gather of each tile cell's three edge slots, R and the local time
 // step (kernel_Flux_e2c gpu_solver.cu:768-878).  Returns this thread's
-// minimum of A*CFL/T (global time step mode).
+// minimum of A*CFL/T (global time step mode).  The local time step is stored
+// as dt in FP64 and as dt / A in FP32 (the RK update's only use of dt is
+// dt R / A, so FP32 folds the division by the area in here and the update
+// reads no area).  ar0: this thread's first cell's (1 / A, A), read at kernel
+// start.
 template <typename R>
 __device__ __forceinline__ R gather_cells(const TiledArgs<R>& a, const TileDesc& d, const FlowT<R>& f,
                                           const SVec4<R>* sF, const R* sS, SVec4<R>* Rout, R* dtout,
-                                          const int2& sl0, R area0) {
+                                          const int2& sl0, Pair<R> ar0) {
   const int s0 = threadIdx.x;
   R tmin = R(3.0e38);
   for (int i = threadIdx.x; i < d.ncell; i += blockDim.x) {
@@ -448,13 +468,17 @@ __device__ __forceinline__ R gather_cells(const TiledArgs<R>& a, const TileDesc&
       r0 += sg * F.d; r1 += sg * F.du; r2 += sg * F.dv; r3 += sg * F.de;
     }
     Rout[d.st0 + g] = SVec4<R>{r0, r1, r2, r3};
-    const R area = (i == s0) ? area0 : a.cg[d.geo0 + g].area;
+    const Pair<R> ar = (i == s0) ? ar0 : area_pair(a.cg + d.geo0, g);
     if (a.global_dt) {
-      const R v = area * f.cfl / T;
+      const R v = ar.b * f.cfl / T;
       dtout[d.st0 + g] = v;
       tmin = fmin(tmin, v);
     } else {
-      dtout[d.st0 + g] = fmin(area * f.cfl / T, R(0.1));
+      const R dt = fmin(ar.b * f.cfl / T, R(0.1));
+      if constexpr (sizeof(R) == 4)
+        dtout[d.st0 + g] = dt * ar.a;
+      else
+        dtout[d.st0 + g] = dt;
     }
   }
   return tmin;
@@ -507,10 +531,10 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
   EdgeRec<R> e0{};
   if (s0 < d.nedge) e0 = a.edges[d.edge0 + s0];
   int2 sl0 = make_int2(0, 0);
-  R area0 = R(0);
+  Pair<R> ar0{R(0), R(0)};
   if (s0 < nt) {
     sl0 = a.slot[d.geo0 + d.cell0 + s0];
-    area0 = a.cg[d.geo0 + d.cell0 + s0].area;
+    ar0 = area_pair(a.cg + d.geo0, d.cell0 + s0);
   }
 
   for (int j = threadIdx.x; j < nl; j += blockDim.x) {
@@ -529,27 +553,28 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
   __syncthreads();
   flux_edges<R>(a, d, f, sW, sG, sB, sF, sS, ML, e0);
   __syncthreads();
-  const R tmin = gather_cells<R>(a, d, f, sF, sS, a.res, a.dt, sl0, area0);
+  const R tmin = gather_cells<R>(a, d, f, sF, sS, a.res, a.dt, sl0, ar0);
   if (a.global_dt) {
-    __shared__ R red[512];
-    red[threadIdx.x] = tmin;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      R m = red[0];
-      for (int k = 1; k < (int)blockDim.x; k++) m = fmin(m, red[k]);
-      a.tile_min[t] = m;
-    }
+    const R m = block_min(tmin);
+    if (threadIdx.x == 0) a.tile_min[t] = m;
   }
 }
 
-// Per-case minimum of the tile minima (global time step mode).
+// Per-case minimum of the tile minima (global time step mode,
+// kernel_TimeStep gpu_solver.cu:880-936 with the host's minimum over its
+// blocks): one workgroup per case, each thread the minimum of a strided
+// share of the tiles, then block_min.  The starting value 9999999 is the
+// reference's.
+constexpr int kDtMinThreads = 256;
 template <typename R>
-__global__ void global_dt_min(const CaseCtl* ctl, const R* tile_min, R* dt_global, int ncase) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncase || !ctl[c].active) return;
+__global__ void __launch_bounds__(kDtMinThreads) global_dt_min(const CaseCtl* ctl, const R* tile_min,
+                                                               R* dt_global) {
+  const CaseCtl& c = ctl[blockIdx.x];
+  if (!c.active) return;  // read by every thread; no other launch writes it meanwhile
   R m = R(9999999.0);
-  for (int k = 0; k < ctl[c].ntile; k++) m = fmin(m, tile_min[ctl[c].tile0 + k]);
-  dt_global[c] = m;
+  for (int k = threadIdx.x; k < c.ntile; k += blockDim.x) m = fmin(m, tile_min[c.tile0 + k]);
+  m = block_min(m);
+  if (threadIdx.x == 0) dt_global[blockIdx.x] = m;
 }
 
 // --------------------------------------------------------------------------
@@ -615,13 +640,18 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
     }
     const R dtc = a.global_dt ? a.dt_global[d.icase] : a.dt[d.st0 + g];
     const R rk = a.rk * dtc;
-    const R area = a.cg[d.geo0 + g].area;
     const SVec4<R> w0 = a.w0[d.st0 + g];
     SVec4<R> wn;
-    wn.d = w0.d - rk * ri.d / area;
-    wn.du = w0.du - rk * ri.du / area;
-    wn.dv = w0.dv - rk * ri.dv / area;
-    wn.de = w0.de - rk * ri.de / area;
+    if (sizeof(R) == 4 && !a.global_dt) {
+      // FP32 local time step: dt / A (see gather_cells)
+      wn = SVec4<R>{w0.d - rk * ri.d, w0.du - rk * ri.du, w0.dv - rk * ri.dv, w0.de - rk * ri.de};
+    } else {
+      const R area = a.cg[d.geo0 + g].area;
+      wn.d = w0.d - rk * ri.d / area;
+      wn.du = w0.du - rk * ri.du / area;
+      wn.dv = w0.dv - rk * ri.dv / area;
+      wn.de = w0.de - rk * ri.de / area;
+    }
     a.w_out[d.st0 + g] = wn;
     if (a.last_stage) {
       const double df = (double)wn.d - (double)w0.d;
@@ -656,12 +686,22 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   const int t = tile_of_block(blockIdx.x, a.ntiles);
   const TileDesc d = a.tiles[t];
   const CaseCtl& cc = a.ctl[d.icase];
-  // the first launch of a run prepares every healthy case; later ones only
-  // the cases still iterating.  With one case the test is made once the
-  // previous stage's residual has been requested (below), so that those
-  // loads do not wait for the control block; the host stops launching once
-  // the case stops.  With several, a stopped case's tiles leave at once.
-  if (a.apply_update ? (a.early_exit && !cc.active) : cc.status != FVM_OK) return;
+  // The first launch of a run prepares every healthy case; later ones only
+  // the cases still iterating.  `active` may change during a launch that
+  // closes the previous step (the workgroup of the case's first tile writes
+  // it at its end), so thread 0 reads it once and the workgroup acts on that
+  // one value, shared through LDS.  With one case the flag is read once the
+  // previous stage's residual has been requested (below), so that those loads
+  // do not wait for the control block; the host stops launching once the
+  // case stops.  With several, a stopped case's tiles leave before any load.
+  __shared__ int s_active;
+  if (!a.apply_update) {
+    if (cc.status != FVM_OK) return;  // no launch writes it meanwhile
+  } else if (a.early_exit) {
+    if (threadIdx.x == 0) s_active = cc.active;
+    __syncthreads();
+    if (!s_active) return;
+  }
   const FlowT<R> f = a.flow[d.icase];
   const int nt = d.ncell, NG = nt + d.nhalo, NW = NG + d.n2, NR = NW + d.n3, N4 = NR + d.n4;
   const int M4 = a.max_r4_local, MR = a.max_ring_local, MW = a.max_w_local, ME = a.max_edges;
@@ -690,10 +730,10 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   EdgeRec<R> e0{};
   if (kPreEdge && s0 < d.nedge) e0 = a.edges[d.edge0 + s0];
   int2 sl0 = make_int2(0, 0);
-  R area0 = R(0);
+  Pair<R> ar0{R(0), R(0)};
   if (s0 < nt) {
     sl0 = a.slot[d.geo0 + d.cell0 + s0];
-    area0 = cg[d.cell0 + s0].area;
+    ar0 = area_pair(cg, d.cell0 + s0);
   }
 
   // this thread's first local-neighbour entry, read here with the other
@@ -723,8 +763,9 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
       sR[j] = Rin[g];
       if (j < NW) sGid[j] = g;
     }
-    if (!cc.active) return;  // the same for the whole workgroup
+    if (!a.early_exit && threadIdx.x == 0) s_active = cc.active;
     __syncthreads();
+    if (!s_active) return;
     // 2. smoothing sweep 1 on the tile and rings 1..3
     if (a.smoothing) {
       for (int j = threadIdx.x; j < NR; j += blockDim.x) {
@@ -753,12 +794,14 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     // 3. sweep 2, selection and the RK update on the tile and rings 1..2
     for (int j = threadIdx.x; j < NW; j += blockDim.x) {
       const int g = sGid[j];
-      const R dtv = a.dt_in[d.st0 + g];
-      R area;
-      if (j == s0 && j < nt)
-        area = area0;  // this thread's tile cell, read at kernel start
-      else
-        area = cg[g].area;
+      const R dtv = a.dt_in[d.st0 + g];  // FP32: dt / A
+      R area = R(1);
+      if constexpr (sizeof(R) == 8) {
+        if (j == s0 && j < nt)
+          area = ar0.b;  // this thread's tile cell, read at kernel start
+        else
+          area = cg[g].area;
+      }
       const SVec4<R> w0 = W0[g];
       SVec4<R> ri = sR[j];
       if (a.smoothing) {
@@ -787,10 +830,14 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
       }
       const R rk = a.rk * dtv;
       SVec4<R> wn;
-      wn.d = w0.d - rk * ri.d / area;
-      wn.du = w0.du - rk * ri.du / area;
-      wn.dv = w0.dv - rk * ri.dv / area;
-      wn.de = w0.de - rk * ri.de / area;
+      if constexpr (sizeof(R) == 4) {
+        wn = SVec4<R>{w0.d - rk * ri.d, w0.du - rk * ri.du, w0.dv - rk * ri.dv, w0.de - rk * ri.de};
+      } else {
+        wn.d = w0.d - rk * ri.d / area;
+        wn.du = w0.du - rk * ri.du / area;
+        wn.dv = w0.dv - rk * ri.dv / area;
+        wn.de = w0.de - rk * ri.de / area;
+      }
       sW[j] = wn;
       if (a.close_step && j < nt) {
         W0n[g] = wn;
@@ -834,7 +881,7 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   flux_edges<R, kPreEdge>(a, d, f, sW, sG, sB, sF, sS, ML, e0);
   __syncthreads();
   // 6. ordered gather: this stage's residual and time step
-  gather_cells<R>(a, d, f, sF, sS, a.res, a.dt, sl0, area0);
+  gather_cells<R>(a, d, f, sF, sS, a.res, a.dt, sl0, ar0);
   // the previous step's close, by the workgroup of the case's first tile once
   // its own tile is done.  The other tiles of this launch read `active` only
   // (a tile that sees the old value computes a residual that the next launch,

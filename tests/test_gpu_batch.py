@@ -59,18 +59,40 @@ def test_per_case_stop(meshes):
 
 
 def test_resume_equals_one_run(meshes):
-    # the second call needs a longer history (a new buffer under the same
-    # captured graphs), the third fits in the one it got
+    # a batch holds 4096 steps of residual history from its creation; the
+    # second call needs a longer one (a new buffer under the same captured
+    # graphs), the third fits in the one it got
     a = Solver(meshes[0], precision="fp32", **FLOWS[1])
     a.run(30)
-    h1a, _, _ = a.run(50)
-    h1, n1, _ = a.run(60)
+    h1a, _, _ = a.run(4150)
+    h1, n1, _ = a.run(4200)
     b = Solver(meshes[0], precision="fp32", **FLOWS[1])
-    h2, n2, _ = b.run(60)
-    assert n1 == n2 == 60
+    h2, n2, _ = b.run(4200)
+    assert n1 == n2 == 4200
     assert np.array_equal(a.state(), b.state())
-    np.testing.assert_array_equal(h1a[31:], h2[31:51])
-    np.testing.assert_array_equal(h1[51:], h2[51:])
+    np.testing.assert_array_equal(h1a[31:], h2[31:4151])
+    np.testing.assert_array_equal(h1[4151:], h2[4151:])
+
+
+def test_run_lengths_cover_graph_ladder(meshes):
+    # a run is whole graphs of steps_per_graph steps, graphs of 2^k steps and
+    # at most one direct step; any split of a run into calls gives the same
+    # bits as one call
+    for precision, kernels in (("fp32", "tiled"), ("fp64", "tiled"), ("fp64", "split"),
+                               ("fp64", "plain")):
+        a = Solver(meshes[0], precision=precision, kernels=kernels, **FLOWS[2])
+        hist = []
+        for n in (1, 4, 9, 30, 79, 127, 128):   # increments 1, 3, 5, 21, 49, 48, 1
+            h, k, st = a.run(n)
+            assert k == n and st == 0
+            hist.append(h)
+        b = Solver(meshes[0], precision=precision, kernels=kernels, **FLOWS[2])
+        hb, _, _ = b.run(128)
+        assert np.array_equal(a.state(), b.state()), (precision, kernels)
+        prev = 0
+        for n, h in zip((1, 4, 9, 30, 79, 127, 128), hist):
+            np.testing.assert_array_equal(h[prev + 1:n + 1], hb[prev + 1:n + 1])
+            prev = n
 
 
 def test_state_round_trip_and_restart(meshes):

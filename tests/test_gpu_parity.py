@@ -169,3 +169,51 @@ def test_fp64_bitwise_on_reference_7k_mesh():
         assert st == 0 and n == steps
         assert np.array_equal(s.state(), Wo), kernels
         np.testing.assert_allclose(hist[1:], hist_o[1:steps + 1], rtol=1e-12, atol=0)
+
+
+def supersonic_far_edges(mesh, mach, alpha, t_in):
+    """Far-field edges on which the reference takes the supersonic branch
+    (gpu_solver.cu:79: |u_inf . n| / AIN > 1 with the dimensional AIN) at the
+    freestream initial state: (inflow count, outflow count)."""
+    om = OracleMesh(mesh.x, mesh.y, mesh.tri)
+    far = np.flatnonzero(om.elog == 1)
+    nx, ny = om.vy[far] / om.vn[far], om.vx[far] / om.vn[far]
+    a = np.radians(alpha)
+    un = np.cos(a) * nx + np.sin(a) * ny
+    ain = np.sqrt(1.4 * 287.06 * t_in)
+    sup = np.abs(un) / ain > 1.0
+    return int(np.sum(sup & (un <= 0))), int(np.sum(sup & (un > 0)))
+
+
+@pytest.mark.parametrize("kernels", ["tiled", "split", "plain"])
+def test_supersonic_far_field_bitwise_vs_oracle(naca3k, kernels):
+    # AIN is dimensional (sqrt(gamma R TIN)) while the normal speed is not, so
+    # a small TIN makes AIN < 1 and sends the far-field edges most aligned
+    # with the flow through both supersonic branches: inflow takes the
+    # freestream, outflow the cell's own state (gpu_solver.cu:79-94).  TIN
+    # leaves the non-dimensional freestream unchanged (p = 1 / (gamma M^2)).
+    t_in = 0.002
+    nin, nout = supersonic_far_edges(naca3k, 0.8, 1.25, t_in)
+    assert nin > 0 and nout > 0, (nin, nout)
+    steps = 40
+    orc, hist_o = oracle_run(naca3k, steps, 0.8, 1.25, 1.5, t_in=t_in)
+    s = Solver(naca3k, 0.8, 1.25, cfl=1.5, t_in=t_in, precision="fp64", kernels=kernels)
+    hist, n, st = s.run(steps)
+    assert st == 0 and n == steps
+    assert np.array_equal(s.state(), orc.state())
+    np.testing.assert_allclose(hist[1:], hist_o[1:steps + 1], rtol=1e-12, atol=0)
+    # the branch changes the solution: the subsonic boundary gives another state
+    sub = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp64", kernels=kernels)
+    sub.run(steps)
+    assert not np.array_equal(sub.state(), s.state())
+
+
+def test_fp32_global_dt_close_to_oracle(rae3k):
+    # the global time step (kernel_TimeStep, gpu_solver.cu:880-936) in FP32:
+    # the two-kernel path with the wave-shuffle minima, against the oracle
+    orc, hist_o = oracle_run(rae3k, 20, 0.729, 2.31, 1.0, order=2, smoothing=True, global_dt=True)
+    s = Solver(rae3k, 0.729, 2.31, cfl=1.0, global_dt=True, precision="fp32")
+    hist, n, st = s.run(20)
+    assert st == 0 and n == 20
+    assert rel_l2(s.state()[:, 0], orc.state()[:, 0]) < 1e-4
+    np.testing.assert_allclose(hist[1:], hist_o[1:21], rtol=1e-3)

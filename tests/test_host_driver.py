@@ -124,3 +124,45 @@ def test_driver_node_averages_and_wall_pressure(driver_run):
     got = np.array([[float(v) for v in l.split()] for l in lines[3:3 + 2 * len(wall)]])
     np.testing.assert_allclose(got[:, 0], mesh.x[pts], rtol=0, atol=1.5e-10)
     np.testing.assert_allclose(got[:, 1], cp[pts], rtol=0, atol=1.5e-10)
+
+
+@pytest.mark.gpu
+def test_driver_forces_match_integrated_wall_pressure(driver, tmp_path):
+    # The coefficients the program prints (fvm_get_forces: the cells' wall
+    # fluxes) against the integral of the wall pressure coefficient it writes
+    # in the reference's format (unairfoilpress.plt: node averages of
+    # Node_Computing, gpu_fvm.cu:182-185), on a converged subsonic first-order
+    # case.  Two discretisations of one surface integral: they agree to a few
+    # per cent, while a force of the wrong sign or from the wrong components
+    # would be off by its whole size.
+    from force_checks import cp_force_coefficients, wall_edges
+    from gpueularsolver2d_amd import mesh_topology, naca0012_mesh, write_dat
+    mesh = naca0012_mesh(3000, seed=1)
+    write_dat(mesh, str(tmp_path / "naca.dat"))
+    (tmp_path / "input.dat").write_text("0.5 2.0 1.4 101325 288.15 1.5 1e-7 20000\n")
+    r = subprocess.run([driver, "naca.dat", "input.dat", "--order", "1", "--no-smooth", "--quiet"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    line = [l for l in r.stdout.splitlines() if l.startswith("steps ")][-1].split()
+    cl, cd = float(line[line.index("Cl") + 1]), float(line[line.index("Cd") + 1])
+    assert float(line[3]) < 1e-7   # converged
+    t = mesh_topology(mesh)
+    x, y, tri = np.asarray(mesh.x), np.asarray(mesh.y), np.asarray(mesh.tri) - 1
+    cx = np.concatenate([[0.0], x[tri].mean(axis=1)])   # 1-based cell ids
+    cy = np.concatenate([[0.0], y[tri].mean(axis=1)])
+    a, b, _ = wall_edges(t, x, y, cx, cy)
+    lines = open(tmp_path / "unairfoilpress.plt").read().splitlines()
+    got = np.array([[float(v) for v in l.split()] for l in lines[3:3 + 2 * len(a)]])
+    # the file lists (x, -Cp) for node1 and node2 of each wall edge in wall-id order
+    wall = np.flatnonzero(t["elog"] == 2)
+    wall = wall[np.argsort(t["wallid"][wall])]
+    n1, n2 = t["node1"][wall] - 1, t["node2"][wall] - 1
+    np.testing.assert_allclose(got[0::2, 0], x[n1], atol=1.5e-10)
+    cp = np.zeros(len(x))
+    cp[n1] = -got[0::2, 1]
+    cp[n2] = -got[1::2, 1]
+    clp, cdp = cp_force_coefficients(x, y, a, b, cp, 2.0)
+    print(f"printed Cl {cl:.5f} Cd {cd:.5f}; integrated wall Cp: Cl {clp:.5f} Cd {cdp:.5f}", flush=True)
+    assert cl > 0.1
+    assert abs(clp - cl) < 0.06 * abs(cl)
+    assert abs(cdp - cd) < 0.015

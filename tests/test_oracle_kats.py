@@ -114,6 +114,34 @@ def test_far_ghost_outflow_pressure(solver):
     assert g[2] / g[0] == pytest.approx(0.1, rel=1e-13)  # tangential velocity kept
 
 
+def test_far_ghost_supersonic_branches():
+    # |u_inf . n| / AIN > 1 with the dimensional AIN = sqrt(gamma R TIN)
+    # (gpu_solver.cu:79): TIN = 0.002 K gives AIN = 0.897, so an edge whose
+    # normal is within ~26 degrees of the flow direction is supersonic.
+    # Inflow (the cell's normal speed <= 0) takes the freestream state,
+    # outflow the cell's own state, whatever the cell holds (:81-93).
+    x = np.array([0.0, 1.0, 0.0]); y = np.array([0.0, 0.0, 1.0])
+    s = OracleSolver(OracleMesh(x, y, np.array([[1, 2, 3]])), 0.5, 0.0, t_in=0.002)
+    fs = s.freestream()
+    assert fs["AIN"] == pytest.approx(math.sqrt(G * 287.06 * 0.002), rel=1e-15) and fs["AIN"] < 1
+    rhoE_inf = 287.06 * 0.002 / fs["VIN"] ** 2 / (G - 1) + 0.5
+    U_out = cons(1.3, 0.4, -0.2, 2.1)   # moves along +x
+    U_in = cons(0.8, -0.3, 0.1, 1.7)    # moves along -x
+    # n = (vy, vx) / |v| = (1, 0): u_inf . n = 1 > AIN
+    g = far(s, U_out, 0.0, 2.0)
+    assert np.array_equal(g, U_out)
+    g = far(s, U_in, 0.0, 2.0)
+    assert g[0] == 1.0 and g[1] == fs["UI"] and g[2] == fs["VI"] and g[3] == rhoE_inf
+    # n = (-1, 0) (an inflow edge for the freestream): the cell's own motion decides
+    assert np.array_equal(far(s, U_in, 0.0, -2.0), U_in)
+    g = far(s, U_out, 0.0, -2.0)
+    assert g[0] == 1.0 and g[1] == fs["UI"] and g[3] == rhoE_inf
+    # a normal at 60 degrees to the flow is subsonic there (0.5 < AIN): the
+    # characteristic branch, which does not return the cell state unchanged
+    g = far(s, U_out, math.sin(math.radians(60)), math.cos(math.radians(60)))
+    assert not np.array_equal(g, U_out)
+
+
 def test_wall_ghost_mirror_and_zero_mass_flux():
     U = cons(1.0, 0.7, 0.2, 1.5)
     for th in (0.3, 1.9, 4.4):
