@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -25,6 +26,33 @@ namespace fvm {
 
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+// Device memory of fvm_run_noalloc: the step closes' partial sums and the
+// residual history of one run, in a file-scope array so that the entry point
+// allocates nothing.  Each device that runs the library holds its own copy,
+// at its own address, until the process ends.  Host code never uses the
+// array's name as an address (on the host it only names the symbol): the
+// device address comes from hipGetSymbolAddress, once per device, with that
+// device current, and is kept in g_scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[FVM_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// The address of `scratch` on `device`, which must be the current device.
+static void* scratch_on(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    throw Error{FVM_ERR_NO_DEVICE, "device ordinal beyond the library's table"};
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Error{FVM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
 
 #define HIP_OK(expr)                                                                   \
   do {                                                                                 \
@@ -177,6 +205,12 @@ class Batch {
         const fvm_flow_params* flows, const fvm_scheme_opts& o);
   ~Batch();
   void run(int max_steps, double epsl, double* hist, int* steps_done, int* case_status);
+  // run() with the step closes' partial sums and the residual history staged
+  // in `scratch` (device memory the caller provides, scratch_bytes long)
+  // instead of the batch's own buffers, launched without graphs: allocates
+  // no memory on the device or the host.
+  void run_in_scratch(int max_steps, double epsl, double* hist, int* steps_done, int* case_status,
+                      void* scratch, size_t scratch_bytes);
   void get_state(int icase, double* rho, double* rhou, double* rhov, double* rhoe);
   void set_state(int icase, const double* rho, const double* rhou, const double* rhov,
                  const double* rhoe);
@@ -185,6 +219,7 @@ class Batch {
   double last_ms = 0.0;
   int64_t last_csu = 0;
   int ncase() const { return (int)cases_.size(); }
+  int device() const { return device_; }
   int ncell(int icase) const { return topo_[cases_[icase].mesh].ncell; }
   void config(int32_t cfg[6]) const {
     cfg[0] = tile_cells_;
@@ -275,6 +310,7 @@ class Batch {
   int hist_stride_ = 0;
   HistDesc* d_hdesc_ = nullptr;   // where the step closes write it (device)
   HistDesc h_hdesc_{nullptr, 0, 0};
+  HistDesc h_scratch_desc_{nullptr, 0, 0};  // the descriptor run_in_scratch copies to the device
   std::vector<double*> old_hist_; // replaced history buffers, freed with the batch
   std::vector<int> state_off_;   // per case: offset in the per-cell state arrays (tiled)
   // tiled buffers (type-erased; R = float or double)
@@ -363,6 +399,8 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
     throw Error{FVM_ERR_NO_DEVICE, "device ordinal " + std::to_string(o.device) + " out of range"};
   device_ = o.device;
   HIP_OK(hipSetDevice(device_));
+  // set-up of fvm_run_noalloc's scratch on this device (once per device)
+  (void)scratch_on(device_);
   HIP_OK(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device_));
   fused_ = o.kernels == FVM_KERNELS_TILED && !o.global_dt;
   if (o.kernels != FVM_KERNELS_PLAIN) {
@@ -1130,15 +1168,16 @@ void Batch::ensure_history(int64_t rows) {
   HIP_OK(hipStreamSynchronize(stream_));
 }
 
-// The step loop of a run: every active case advances one step per step
-// launched.  The remaining steps of the least advanced active case are split
-// into whole chunks of steps_per_graph_ steps, then chunks of 2^k steps, then
-// at most one single step; launch(n, i) queues chunk i of n steps and returns
-// the pinned slot where the chunk's last step close leaves the control
-// blocks.  Chunks are queued one ahead: while chunk i runs, the host reads
-// chunk i - 1's slot and stops queueing once no case is active (a chunk
-// queued after the last case stopped only runs launches that return at
-// once).  Consecutive chunks use different slots.  Allocates nothing.
+// The step loop of a run (run() and run_in_scratch()): every active case
+// advances one step per step launched.  The remaining steps of the least
+// advanced active case are split into whole chunks of steps_per_graph_ steps,
+// then chunks of 2^k steps, then at most one single step; launch(n, i) queues
+// chunk i of n steps and returns the pinned slot where the chunk's last step
+// close leaves the control blocks.  Chunks are queued one ahead: while chunk
+// i runs, the host reads chunk i - 1's slot and stops queueing once no case is
+// active (a chunk queued after the last case stopped only runs launches that
+// return at once).  Consecutive chunks use different slots.  Allocates
+// nothing.
 template <typename Launch>
 void Batch::run_steps(int max_steps, double epsl, Launch launch) {
   const int ncase = (int)cases_.size();
@@ -1262,6 +1301,50 @@ void Batch::sync_ctl() {
   HIP_OK(hipMemcpyAsync(d_ctl_, io, cases_.size() * sizeof(CaseCtl), hipMemcpyHostToDevice, stream_));
   HIP_OK(hipStreamSynchronize(stream_));
   ctl_on_host_ = false;
+}
+
+// Scratch layout: the history descriptor (256 bytes), the partial sums
+// (rounded up to 256 bytes), then ncase rows of max_steps + 1 doubles.
+static size_t run_scratch_bytes(int n_partial, int ncase, int max_steps) {
+  const size_t part = ((size_t)n_partial * sizeof(double) + 255) & ~size_t(255);
+  return 256 + part + (size_t)ncase * ((size_t)max_steps + 1) * sizeof(double);
+}
+
+void Batch::run_in_scratch(int max_steps, double epsl, double* hist, int* steps_done,
+                           int* case_status, void* scratch, size_t scratch_bytes) {
+  HIP_OK(hipSetDevice(device_));
+  if (max_steps < 0) throw Error{FVM_ERR_ARGUMENT, "max_steps must be >= 0"};
+  if (max_steps == INT32_MAX) throw Error{FVM_ERR_ARGUMENT, "max_steps must be < 2^31 - 1"};
+  const int ncase = (int)cases_.size();
+  const size_t need = run_scratch_bytes(n_partial_, ncase, max_steps);
+  if (need > scratch_bytes)
+    throw Error{FVM_ERR_ARGUMENT, "fvm_run_noalloc: this run needs " + std::to_string(need) +
+                                      " bytes of scratch (partial sums and residual history), "
+                                      "more than the " + std::to_string(scratch_bytes) +
+                                      " the library holds; use fvm_run or fewer steps"};
+  char* base = static_cast<char*>(scratch);
+  HistDesc* hdesc = reinterpret_cast<HistDesc*>(base);
+  double* partial = reinterpret_cast<double*>(base + 256);
+  double* rows = reinterpret_cast<double*>(
+      base + 256 + (((size_t)n_partial_ * sizeof(double) + 255) & ~size_t(255)));
+  h_scratch_desc_ = HistDesc{rows, max_steps + 1, 0};
+  HIP_OK(hipMemcpyAsync(hdesc, &h_scratch_desc_, sizeof(HistDesc), hipMemcpyHostToDevice, stream_));
+  run_steps(max_steps, epsl, [&](int n, int i) {
+    CaseCtl* slot = pin_slot(kDirectSlot + (i & 1));
+    enqueue_steps(n, RunBufs{partial, hdesc, slot});
+    return slot;
+  });
+  for (int c = 0; c < ncase; c++) {
+    const int s0 = before_[c] + 1, s1 = std::min(h_ctl_[c].step, max_steps);
+    double* row = hist ? hist + (size_t)c * ((size_t)max_steps + 1) : nullptr;
+    if (row) {
+      row[0] = 100.0;
+      if (s1 >= s0)
+        HIP_OK(hipMemcpy(row + s0, rows + (size_t)c * ((size_t)max_steps + 1) + s0,
+                         (size_t)(s1 - s0 + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    }
+  }
+  finish_run(steps_done, case_status);
 }
 
 void Batch::get_state(int icase, double* rho, double* rhou, double* rhov, double* rhoe) {
@@ -1563,6 +1646,21 @@ fvm_status fvm_run(fvm_solver* s, int32_t max_steps, double epsl, double* resid_
   fvm_status st = guard([&] {
     if (!s) throw Error{FVM_ERR_ARGUMENT, "NULL solver"};
     s->b->run(max_steps, epsl, resid_hist, steps_done, &status);
+  });
+  if (st != FVM_OK) return st;
+  if (status != FVM_OK) fvm::set_last_error("the residual became NaN");
+  return (fvm_status)status;
+}
+
+fvm_status fvm_run_noalloc(fvm_solver* s, int32_t max_steps, double epsl, double* resid_hist,
+                           int32_t* steps_done) {
+  int32_t status = FVM_OK;
+  fvm_status st = guard([&] {
+    if (!s) throw Error{FVM_ERR_ARGUMENT, "NULL solver"};
+    HIP_OK(hipSetDevice(s->b->device()));
+    void* p = fvm::scratch_on(s->b->device());
+    s->b->run_in_scratch(max_steps, epsl, resid_hist, steps_done, &status, p,
+                         FVM_NOALLOC_SCRATCH_BYTES);
   });
   if (st != FVM_OK) return st;
   if (status != FVM_OK) fvm::set_last_error("the residual became NaN");

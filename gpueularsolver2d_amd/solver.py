@@ -34,7 +34,7 @@ EXPORTS = ("fvm_default_opts", "fvm_status_string", "fvm_last_error", "fvm_mesh_
            "fvm_get_timing", "fvm_destroy", "fvm_batch_create", "fvm_batch_run",
            "fvm_batch_get_state", "fvm_batch_set_state", "fvm_batch_get_forces",
            "fvm_batch_get_timing", "fvm_batch_get_config", "fvm_get_config",
-           "fvm_batch_kernel_times", "fvm_kernel_times", "fvm_batch_destroy")
+           "fvm_batch_kernel_times", "fvm_kernel_times", "fvm_batch_destroy", "fvm_run_noalloc")
 
 
 class FvmError(RuntimeError):
@@ -85,6 +85,7 @@ def load_library(path=LIB_PATH):
     L.fvm_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(FlowParams), C.POINTER(SchemeOpts),
                              C.POINTER(C.c_void_p)]
     L.fvm_run.argtypes = [C.c_void_p, C.c_int32, C.c_double, _dp, _ip]
+    L.fvm_run_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_double, _dp, _ip]
     L.fvm_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     L.fvm_set_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     L.fvm_get_forces.argtypes = [C.c_void_p, _dp, _dp]
@@ -105,7 +106,8 @@ def load_library(path=LIB_PATH):
     L.fvm_kernel_times.argtypes = [C.c_void_p, C.c_int32, _dp]
     L.fvm_batch_destroy.argtypes = [C.c_void_p]
     L.fvm_batch_destroy.restype = None
-    for name in ("fvm_mesh_topology", "fvm_create", "fvm_run", "fvm_get_state", "fvm_set_state",
+    for name in ("fvm_mesh_topology", "fvm_create", "fvm_run", "fvm_run_noalloc", "fvm_get_state",
+                 "fvm_set_state",
                  "fvm_get_forces", "fvm_get_timing", "fvm_batch_create", "fvm_batch_run",
                  "fvm_batch_get_state", "fvm_batch_set_state", "fvm_batch_get_forces",
                  "fvm_batch_get_timing", "fvm_batch_get_config", "fvm_get_config",
@@ -211,6 +213,16 @@ class Solver:
         hist = np.zeros(max_steps + 1)
         steps = C.c_int32()
         st = load_library().fvm_run(self.h, max_steps, epsl, _dptr(hist), C.byref(steps))
+        if check and st not in (0,):
+            _check(st)
+        return hist[: steps.value + 1], steps.value, st
+
+    def run_noalloc(self, max_steps, epsl=-1.0, check=True):
+        """run() through fvm_run_noalloc: no device allocation, scratch held by
+        the library (at most FVM_NOALLOC_SCRATCH_BYTES per device)."""
+        hist = np.zeros(max_steps + 1)
+        steps = C.c_int32()
+        st = load_library().fvm_run_noalloc(self.h, max_steps, epsl, _dptr(hist), C.byref(steps))
         if check and st not in (0,):
             _check(st)
         return hist[: steps.value + 1], steps.value, st

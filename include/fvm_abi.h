@@ -131,6 +131,25 @@ fvm_status fvm_create(const fvm_mesh_desc *mesh, const fvm_flow_params *flow,
 fvm_status fvm_run(fvm_solver *s, int32_t max_steps, double epsl, double *resid_hist,
                    int32_t *steps_done);
 
+/* fvm_run without allocating: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results
+ * and status codes as fvm_run, bit for bit, with two differences:
+ *  - the step closes' per-tile partial sums and the device copy of the
+ *    residual history live in a scratch array of FVM_NOALLOC_SCRATCH_BYTES
+ *    that the library holds on each device (set up by fvm_create), so a run
+ *    may cover at most the steps that fit:
+ *      256 + 8 * tiles (rounded up to 256) + 8 * (max_steps + 1) bytes;
+ *    a larger request returns FVM_ERR_ARGUMENT and runs nothing;
+ *  - the steps are launched directly on the solver's stream, without graphs
+ *    (instantiating a graph allocates).
+ * Every call on a device shares that device's scratch and keeps the last
+ * call's contents there; calls that overlap on one device race, so run one at
+ * a time per device.  Nothing in the scratch needs to start at zero. */
+#define FVM_NOALLOC_SCRATCH_BYTES (4u << 20)
+fvm_status fvm_run_noalloc(fvm_solver *s, int32_t max_steps, double epsl, double *resid_hist,
+                           int32_t *steps_done);
+
 /* Final conservative state per cell (reference cell order), FP64. */
 fvm_status fvm_get_state(fvm_solver *s, double *rho, double *rhou, double *rhov, double *rhoE);
 /* Replaces the state (checkpoint restore); the step count is kept. */
