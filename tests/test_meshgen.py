@@ -50,3 +50,27 @@ def test_truncated_file_rejected(tmp_path):
     p.write_text("10 5\n0 1 2\n")
     with pytest.raises(ValueError):
         read_dat(p)
+
+
+def _digest(m):
+    import hashlib
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(m.x, dtype="<f8").tobytes())
+    h.update(np.ascontiguousarray(m.y, dtype="<f8").tobytes())
+    h.update(np.ascontiguousarray(m.tri, dtype="<i4").tobytes())
+    return h.hexdigest()[:16]
+
+
+@pytest.mark.parametrize("name,cells,nodes,digest", [
+    ("C1", 3024, 1600, "2d90e0a46cb20242"),
+    ("C2", 50740, 25458, "4390848570830f45"),
+    ("C3", 198578, 99377, "32a4cd2e02b5b100"),
+])
+def test_benchmark_meshes_are_pinned(name, cells, nodes, digest):
+    # the configurations' meshes, bit for bit: coordinates and triangles as
+    # generated here (the generator triangulates with scipy's Qhull, so a
+    # different Qhull or numpy build that changed a mesh shows up here)
+    from gpueularsolver2d_amd import workloads
+    m = workloads.config_mesh(getattr(workloads, name))
+    assert (m.ncell, m.nnode) == (cells, nodes)
+    assert _digest(m) == digest
