@@ -307,7 +307,7 @@ def config_c1(steps=2000):
                 dcl_vs_oracle=cl - clo, dcd_vs_oracle=cd - cdo)
 
 
-def config_c2(steps=5000, window=500, every=10):
+def config_c2(steps=5000, window=500, every=5):
     """Config C2: the ~50k-triangle NACA 0012 mesh (seed 2), M 0.8, alpha
     1.25, second order with the limiter and smoothing, CFL 1.5, 5000 steps.
     FP32 runs the launch shape the autotuner picks for this mesh (its table is
@@ -356,7 +356,7 @@ def config_c2(steps=5000, window=500, every=10):
     return out
 
 
-def batch64(steps, window=500, every=10):
+def batch64(steps, window=500, every=5):
     """64 cases in one batch: four NACA 0012 meshes of uneven size (~3k, 10k,
     50k, 90k triangles, seeds 11-14) x 16 (Mach, alpha) pairs on the grid
     M in {0.5, 0.6167, 0.7333, 0.85} x alpha in {0, 1, 2, 3} deg; second order
@@ -382,7 +382,7 @@ def batch64(steps, window=500, every=10):
                 window_accuracy=acc)
 
 
-def batch256(steps, window=500, every=10, workers=16):
+def batch256(steps, window=500, every=5, workers=16):
     """256 cases in one batch, every case on its own mesh: sizes log-uniform in
     [2k, 150k] triangles (mesh seed 1000 + i), M uniform in [0.3, 0.85], alpha
     uniform in [-2, 4] deg (numpy seed 256), second order with the limiter and
@@ -426,7 +426,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--acc-window", type=lambda v: [int(x) for x in v.split(":")],
-                    default=[30000, 70000, 100],
+                    default=[30000, 70000, 20],
                     help="start:end:every of the averaging window of the limit-cycle comparison "
                          "(end 0 skips it)")
     ap.add_argument("--conv-steps", type=int, default=40000,

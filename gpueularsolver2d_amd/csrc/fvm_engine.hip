@@ -1170,8 +1170,8 @@ void Batch::ensure_history(int64_t rows) {
 
 // The step loop of a run (run() and run_in_scratch()): every active case
 // advances one step per step launched.  The remaining steps of the least
-// advanced active case are split into whole chunks of steps_per_graph_ steps,
-// then chunks of 2^k steps, then at most one single step; launch(n, i) queues
+// advanced active case are split into chunks of 2^k steps, whole chunks of
+// steps_per_graph_ steps and at most one single step; launch(n, i) queues
 // chunk i of n steps and returns the pinned slot where the chunk's last step
 // close leaves the control blocks.  Chunks are queued one ahead: while chunk
 // i runs, the host reads chunk i - 1's slot and stops queueing once no case is
@@ -1211,21 +1211,19 @@ void Batch::run_steps(int max_steps, double epsl, Launch launch) {
   if (m0 >= 0) {
     const int remaining = max_steps - m0;
     const int nfull = remaining / steps_per_graph_;
-    int rest = remaining % steps_per_graph_;
-    int k = 1;  // the largest power of two <= rest
-    while (k * 2 <= rest) k *= 2;
+    const int rest = remaining % steps_per_graph_;
+    // the 2^k chunks first, smallest first: the host's work to launch the
+    // next, larger graph then overlaps the device's work on a small one (a
+    // run's first graph launch is otherwise exposed); the single odd step
+    // last, after an even number of steps
+    int ladder[32], nl = 0;
+    for (int k = 2; k <= rest && nl < 32; k *= 2)
+      if (rest & k) ladder[nl++] = k;
+    const int nchunks = nl + nfull + (rest & 1);
     bool any = true;
     CaseCtl* prev = nullptr;
-    for (int i = 0; any; i++) {
-      int n;
-      if (i < nfull) {
-        n = steps_per_graph_;
-      } else {
-        while (k > 1 && (rest & k) == 0) k /= 2;
-        if (rest == 0) break;
-        n = k > 1 ? k : 1;
-        rest -= n;
-      }
+    for (int i = 0; i < nchunks && any; i++) {
+      const int n = i < nl ? ladder[i] : (i < nl + nfull ? steps_per_graph_ : 1);
       CaseCtl* slot = launch(n, i);
       HIP_OK(hipEventRecord(ev_ctl_[i & 1], stream_));
       if (i > 0) {

@@ -117,22 +117,29 @@ def test_fp32_first_order_converged_vs_oracle(naca3k):
 
 def test_fp32_second_order_mean_forces(naca3k):
     # Second order with the per-face limiter settles into a small limit cycle
-    # (the residual stalls), so instantaneous FP32/FP64 states decorrelate;
-    # the time-averaged coefficients must agree.  The FP64 tiled path is the
-    # FP64 oracle bit for bit (test_fp64_bitwise_vs_oracle).
+    # (the residual stalls; Cl cycles with a std of 1.9e-4 here), so
+    # instantaneous FP32/FP64 states decorrelate; the time-averaged
+    # coefficients must agree to the |dCl| < 1e-4 bar.  The mean is taken
+    # every 10 steps over 4000 steps: sampling every 100 steps aliases with the
+    # cycle (20 such samples gave +1.5e-4 where 400 dense ones give -5.9e-5;
+    # FP64 from a 1e-12-perturbed start gives -1.8e-5 and dCd -1.4e-6, FP32
+    # dCd -1.5e-5, measured on MI355X).  The FP64 tiled path is the FP64 oracle
+    # bit for bit (test_fp64_bitwise_vs_oracle).
     s32 = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp32")
     s64 = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp64")
     s32.run(8000)
     s64.run(8000)
     f32, f64 = [], []
-    for k in range(8100, 10001, 100):
+    for k in range(8010, 12001, 10):
         s32.run(k)
         s64.run(k)
         f32.append(s32.forces())
         f64.append(s64.forces())
     f32, f64 = np.array(f32), np.array(f64)
-    assert abs(f32[:, 0].mean() - f64[:, 0].mean()) < 3e-4
-    assert abs(f32[:, 1].mean() - f64[:, 1].mean()) < 1e-4
+    dcl, dcd = f32[:, 0].mean() - f64[:, 0].mean(), f32[:, 1].mean() - f64[:, 1].mean()
+    print(f"window means FP32 - FP64: dCl {dcl:+.2e}, dCd {dcd:+.2e}", flush=True)
+    assert abs(dcl) < 1e-4
+    assert abs(dcd) < 5e-5
 
 
 def test_fp32_short_run_close_to_oracle(rae3k):
