@@ -692,10 +692,7 @@ void Batch::setup_tiled() {
       // FP32 stores the gradient's edge vectors scaled by 0.5 / area (the
       // Green-Gauss face average and the division by the area folded in)
       const double gs = sizeof(R) == 4 ? 0.5 * T.inv_area[i] : 1.0;
-      for (int k = 0; k < 3; k++) {
-        g.vx[k] = (R)(T.gvx[3 * i + k] * gs);
-        g.vy[k] = (R)(T.gvy[3 * i + k] * gs);
-      }
+      for (int k = 0; k < 3; k++) g.set(k, T.gvx[3 * i + k] * gs, T.gvy[3 * i + k] * gs);
       g.inv_area = (R)T.inv_area[i];
       g.area = (R)T.area[i];
       hnbr[geo0[m] + i] = make_int4(T.nbr[3 * i], T.nbr[3 * i + 1], T.nbr[3 * i + 2], T.clog[i]);

@@ -91,9 +91,32 @@ TiledMesh build_tiled(const Topology& topo, int tile_cells);
 // ---------------------------------------------------------------------------
 // Device records and kernel arguments.
 // ---------------------------------------------------------------------------
+// Per cell: the gradient's three local edge vectors (gx, gy of side k) and
+// the area.  FP64 stores all three; FP32 stores two, scaled by 0.5 / area:
+// the three vectors of a triangle's sides sum to zero (x1 - x2 + x2 - x0 +
+// x0 - x1), so the third is minus the sum of the first two.  24 instead of
+// 32 bytes per cell, read for the tile and its first ring every launch.
 template <typename R>
-struct alignas(16) CellGeom {   // per cell: gradient geometry and area
-  R vx[3], vy[3], inv_area, area;  // FP32: vx, vy scaled by 0.5 / area
+struct alignas(16) CellGeom {
+  R vx[3], vy[3], inv_area, area;
+  __host__ __device__ R gx(int k) const { return vx[k]; }
+  __host__ __device__ R gy(int k) const { return vy[k]; }
+  __host__ __device__ void set(int k, double x, double y) {
+    vx[k] = (R)x;
+    vy[k] = (R)y;
+  }
+};
+template <>
+struct alignas(8) CellGeom<float> {
+  float vx[2], vy[2], inv_area, area;
+  __host__ __device__ float gx(int k) const { return k < 2 ? vx[k] : -(vx[0] + vx[1]); }
+  __host__ __device__ float gy(int k) const { return k < 2 ? vy[k] : -(vy[0] + vy[1]); }
+  __host__ __device__ void set(int k, double x, double y) {
+    if (k < 2) {
+      vx[k] = (float)x;
+      vy[k] = (float)y;
+    }
+  }
 };
 
 template <typename R>

@@ -328,14 +328,14 @@ __device__ __forceinline__ void gradient_cell(int j, const Vec4<R>& Wi, const Ce
     if ((nbk[0] | nbk[1] | nbk[2]) < 0) {
 #pragma unroll
       for (int k = 0; k < 3; k++)
-        if (nbk[k] < 0) cs[k] = boundary_ghost<R>(-nbk[k], Wi, cg.vx[k], cg.vy[k], f);
+        if (nbk[k] < 0) cs[k] = boundary_ghost<R>(-nbk[k], Wi, cg.gx(k), cg.gy(k), f);
     }
   }
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     Vec4<R> c;
     if constexpr (sizeof(R) == 4)
-      c = nbk[k] >= 0 ? nbv[k] : boundary_ghost<R>(-nbk[k], Wi, cg.vx[k], cg.vy[k], f);
+      c = nbk[k] >= 0 ? nbv[k] : boundary_ghost<R>(-nbk[k], Wi, cg.gx(k), cg.gy(k), f);
     else
       c = cs[k];
     const R cc[4] = {c.d, c.du, c.dv, c.de};
@@ -358,14 +358,14 @@ __device__ __forceinline__ void gradient_cell(int j, const Vec4<R>& Wi, const Ce
       // FP32: the edge vectors come scaled by 0.5 / area (the ghosts above
       // only use their direction)
 #pragma unroll
-      for (int q = 0; q < 4; q++) gx[q] += (ui[q] + cc[q]) * cg.vy[k];
+      for (int q = 0; q < 4; q++) gx[q] += (ui[q] + cc[q]) * cg.gy(k);
 #pragma unroll
-      for (int q = 0; q < 4; q++) gy[q] += (ui[q] + cc[q]) * cg.vx[k];
+      for (int q = 0; q < 4; q++) gy[q] += (ui[q] + cc[q]) * cg.gx(k);
     } else {
 #pragma unroll
-      for (int q = 0; q < 4; q++) gx[q] += R(0.5) * (ui[q] + cc[q]) * cg.vy[k];
+      for (int q = 0; q < 4; q++) gx[q] += R(0.5) * (ui[q] + cc[q]) * cg.gy(k);
 #pragma unroll
-      for (int q = 0; q < 4; q++) gy[q] += R(0.5) * (ui[q] + cc[q]) * cg.vx[k];
+      for (int q = 0; q < 4; q++) gy[q] += R(0.5) * (ui[q] + cc[q]) * cg.gx(k);
     }
   }
   Pair<R>* sG2 = reinterpret_cast<Pair<R>*>(sG);
