@@ -340,10 +340,10 @@ def config_c2(steps=5000, window=500, every=5):
         sol[name] = s
     A, F = sol["fp64"], sol["fp32"]
     # accuracy: the FP32 and FP64 runs continue for the window, beside FP64
-    # from a perturbed copy of the FP64 state at the start of the window
+    # started from a 1e-12-perturbed freestream and run as long
     P = Solver(m, precision="fp64", **flow)
+    P.set_state(perturbed(P.state(), np.random.default_rng(0)))
     P.run(10 + steps)
-    P.set_state(perturbed(A.state(), np.random.default_rng(0)))
     (cl, cd), (clF, cdF) = A.forces(), F.forces()
     WA, WF = A.state(), F.state()
     out.update(cl_fp64=cl, cd_fp64=cd, dcl=clF - cl, dcd=cdF - cd,
