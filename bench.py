@@ -18,6 +18,7 @@ Measured in the same run:
                    compulsory bytes (98 B, frac_compulsory)
   fp64_tiled / fp64_plain   the same workload in FP64 (tiled kernels; the
                    reference's per-phase kernel structure)
+  global_dt        the same workload with the global time step, FP32 and FP64
   config_c1        3k cells, first order: the GPU FP64 run against the oracle
   config_c2        50k cells, 5000 steps: FP64, FP32 (autotuned launch shape)
                    and plain FP64 throughput; FP32 accuracy as window means
@@ -518,6 +519,16 @@ def main():
     v64 = csu / t64 / 1e6
     vpl = 4.0 * C * plain_steps / tpl / 1e6
     progress(f"fp64 tiled {v64:.1f}, fp64 plain {vpl:.1f} Mcsu/s")
+    # the global time step (kernel_TimeStep's mode): two kernels per stage with
+    # the per-case minimum between them
+    gdt = {}
+    gsteps = max(10, args.steps // 4)
+    for prec in ("fp32", "fp64"):
+        sg = Solver(mesh, precision=prec, global_dt=True, **flow)
+        tg, _ = timed_run(sg, 5, gsteps)
+        kt_g = sg.kernel_times(reps=10)
+        gdt[prec] = dict(value=round(4.0 * C * gsteps / tg / 1e6, 1), steps=gsteps,
+                         flux_tile_ms=kt_g[0], update_tile_ms=kt_g[1])
 
     # the stage kernel's average launch time over the timed steps: the device
     # time of the steps (HIP events on the solver's stream) over their launches
@@ -557,6 +568,7 @@ def main():
                        "kernel_ms_isolated": kt64[0]},
         "fp64_plain": {"value": round(vpl, 1), "steps": plain_steps},
         "fp32_over_fp64_plain": round(v32 / vpl, 2),
+        "global_dt": gdt,
         "config_c1": c1,
         "config_c2": c2,
         "accuracy": acc,
