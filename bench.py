@@ -451,6 +451,9 @@ def main():
                     help="time only the FP32 path (for profiler runs)")
     ap.add_argument("--fp64-only", action="store_true",
                     help="time only the FP64 one-kernel-per-stage path (for profiler runs)")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="time only the FP64 plain path, the reference's kernel structure "
+                         "(for profiler runs)")
     ap.add_argument("--batch256-fp32-only", action="store_true",
                     help="only run the FP32 256-case batch for --batch256-steps steps at the "
                          "library's launch shape (for profiler runs)")
@@ -489,6 +492,12 @@ def main():
     flow = cfg["flow"]
     csu = 4.0 * C * args.steps
 
+    if args.plain_only:
+        sp = Solver(mesh, precision="fp64", kernels="plain", **flow)
+        tpl, _ = timed_run(sp, args.warmup, args.steps)
+        print(json.dumps({"metric": METRIC, "value": round(csu / tpl / 1e6, 1), "unit": "Mcsu/s",
+                          "precision": "fp64", "kernels": "plain", "steps": args.steps}))
+        return 0
     if args.fp64_only:
         s64 = Solver(mesh, precision="fp64", kernels="tiled", **flow)
         t64, _ = timed_run(s64, args.warmup, args.steps)
