@@ -1,6 +1,6 @@
 """Builds libfvm_hip.so (gfx950) in-tree with hipcc, and the host driver.
 
-    python -m gpueularsolver2d_amd.build        # library + fvm_solver driver
+    python -m gpueularsolver2d_amd.build        # library, fvm_solver driver, libmeshgen.so
 
 FP64 translation units are compiled with -ffp-contract=off so every multiply
 and add is rounded separately, matching the FP64 oracle bit for bit; the FP32
@@ -93,9 +93,27 @@ def build_driver(verbose=False):
     return DRIVER
 
 
+MESHGEN = os.path.join(HERE, "libmeshgen.so")
+
+
+def build_meshgen(verbose=False):
+    """The synthetic meshes' Delaunay triangulator (host code, g++)."""
+    src = os.path.join(CSRC, "meshgen_delaunay.cpp")
+    tmp = MESHGEN + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall", src, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"building libmeshgen.so failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, MESHGEN)
+    if verbose:
+        print("built", MESHGEN)
+    return MESHGEN
+
+
 def build(verbose=False):
     build_library(verbose)
     build_driver(verbose)
+    build_meshgen(verbose)
 
 
 if __name__ == "__main__":

@@ -3,9 +3,12 @@
 The workloads are unstructured triangle meshes around a NACA 0012 airfoil
 (thickness 0.12, chord 1): 128 cosine-clustered surface points, a far-field
 circle of radius 20 chords with 48 points, and seeded interior points whose
-spacing grows geometrically with distance from the airfoil.  The points are
-Delaunay-triangulated (Qhull through scipy) and the triangles inside the
-airfoil are removed.
+spacing grows geometrically with distance from the airfoil.  The points,
+rounded to multiples of 2^-40, are Delaunay-triangulated by this package's
+own triangulator (csrc/meshgen_delaunay.cpp: Bowyer-Watson insertion with
+exact integer predicates, built into libmeshgen.so), and the triangles inside
+the airfoil are removed.  The result is put in a canonical order, so it does
+not depend on the order in which the triangulator emits its triangles.
 
 Interior points lie on offset curves of the (convex) airfoil polygon at
 distances d_0 < d_1 < ... with spacing h(d) = h_wall + g*d along and across
@@ -22,8 +25,10 @@ counter-clockwise (positive area, gpu_init.cu:95-112).
 """
 from __future__ import annotations
 
+import ctypes
 import gzip
 import math
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -151,18 +156,67 @@ def _filter_points(px, py, sx, sy, fx, fy):
             if near.any():
                 keep[near] &= np.hypot(px[near] - mx[i], py[near] - my[i]) >= rad[i]
     # keep a clearance from surface vertices of 0.6 local surface spacing
-    from scipy.spatial import cKDTree
+    # (nearest vertex by brute force over the surface, in chunks of points)
     seg = np.hypot(np.roll(sx, -1) - sx, np.roll(sy, -1) - sy)
     local = 0.5 * (seg + np.roll(seg, 1))
-    dist, idx = cKDTree(np.column_stack([sx, sy])).query(np.column_stack([px, py]))
+    dist = np.empty(len(px))
+    idx = np.empty(len(px), dtype=np.int64)
+    for a in range(0, len(px), 8192):
+        dx = px[a:a + 8192, None] - sx[None, :]
+        dy = py[a:a + 8192, None] - sy[None, :]
+        d2 = dx * dx + dy * dy
+        k = np.argmin(d2, axis=1)
+        idx[a:a + 8192] = k
+        dist[a:a + 8192] = np.sqrt(d2[np.arange(len(k)), k])
     keep &= dist >= 0.6 * local[idx]
     return px[keep], py[keep]
 
 
+GRID = 2.0 ** 40  # the triangulator's coordinate grid
+
+
+def snap(v):
+    """Rounds coordinates to the triangulator's grid (multiples of 2^-40)."""
+    return np.round(np.asarray(v, dtype=np.float64) * GRID) / GRID
+
+
+_LIB = None
+
+
+def _meshgen_lib():
+    """libmeshgen.so, built here first if it is missing (g++)."""
+    global _LIB
+    if _LIB is None:
+        from . import build as _build
+        if not os.path.exists(_build.MESHGEN):
+            _build.build_meshgen()
+        lib = ctypes.CDLL(_build.MESHGEN)
+        lib.fvm_meshgen_delaunay.restype = ctypes.c_int
+        lib.fvm_meshgen_delaunay.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_int)]
+        _LIB = lib
+    return _LIB
+
+
+def delaunay(x, y):
+    """Delaunay triangles (counter-clockwise, 0-based node ids) of points on
+    the grid of `snap`, |x|, |y| < 32, whose convex hull edges have empty
+    diametral circles (the generator's boundaries do)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty((2 * len(x) + 8, 3), dtype=np.int32)
+    nt = ctypes.c_int(0)
+    rc = _meshgen_lib().fvm_meshgen_delaunay(len(x), x.ctypes.data, y.ctypes.data, out.ctypes.data,
+                                             len(out), ctypes.byref(nt))
+    if rc != 0:
+        raise RuntimeError({1: "points off the triangulator's grid or out of range",
+                            2: "repeated point", 3: "triangle buffer too small"}.get(rc, f"error {rc}"))
+    return out[:nt.value].astype(np.int64)
+
+
 def _triangulate(x, y, sx, sy, n_surface, n_far):
-    from scipy.spatial import Delaunay
-    pts = np.column_stack([x, y])
-    tri = Delaunay(pts).simplices.astype(np.int64)
+    tri = delaunay(x, y)
     cxs = x[tri].mean(axis=1)
     cys = y[tri].mean(axis=1)
     tri = tri[~_inside_polygon(cxs, cys, sx, sy)]
@@ -215,11 +269,13 @@ def _hilbert_index(x, y, order=16):
 def naca0012_mesh(target_cells, seed, tol=0.02, max_iter=40):
     """Seeded NACA 0012 Delaunay mesh with about `target_cells` triangles.
 
-    Node order: 128 surface points, 48 far-field points, interior points.
-    Triangles are emitted in Hilbert order of their centroids (as a mesh
-    generator with spatial locality would write them)."""
-    sx, sy = naca0012_surface()
-    fx, fy = far_circle()
+    Node order: 128 surface points, 48 far-field points, interior points, all
+    on the triangulator's grid.  Each triangle starts at its smallest node id
+    (counter-clockwise), and the triangles are emitted in Hilbert order of
+    their centroids (as a mesh generator with spatial locality would write
+    them), ties broken by node ids."""
+    sx, sy = (snap(v) for v in naca0012_surface())
+    fx, fy = (snap(v) for v in far_circle())
     seg = np.hypot(np.roll(sx, -1) - sx, np.roll(sy, -1) - sy)
     h_wall = float(seg.mean())
     want_int = max(0, (int(target_cells) - (N_SURFACE + N_FAR)) // 2)
@@ -251,16 +307,20 @@ def naca0012_mesh(target_cells, seed, tol=0.02, max_iter=40):
         if abs(len(px) - want_int) <= tol * max(want_int, 1) or len(px) == 0:
             break
         keep = max(0.05, len(px) / max(len(rx), 1))
-    px, py = best
+    px, py = (snap(v) for v in best)
     x = np.concatenate([sx, fx, px])
     y = np.concatenate([sy, fy, py])
     tri = _triangulate(x, y, sx, sy, N_SURFACE, N_FAR)
+    # smallest node id first, the counter-clockwise order kept
+    first = np.argmin(tri, axis=1)
+    rows = np.arange(len(tri))
+    tri = np.stack([tri[rows, (first + k) % 3] for k in range(3)], axis=1)
     cx = x[tri].mean(axis=1)
     cy = y[tri].mean(axis=1)
     # Hilbert key on a log-polar map keeps the dense near-wall region resolved
     r = np.hypot(cx - CENTER[0], cy - CENTER[1])
     key = _hilbert_index(np.log(r + 0.05), np.arctan2(cy - CENTER[1], cx - CENTER[0]))
-    tri = tri[np.argsort(key, kind="stable")]
+    tri = tri[np.lexsort((tri[:, 2], tri[:, 1], tri[:, 0], key))]
     return Mesh(x=x, y=y, tri=(tri + 1).astype(np.int32), seed=seed,
                 name=f"naca0012-{len(tri)}-s{seed}")
 

@@ -2,8 +2,8 @@
 import numpy as np
 import pytest
 
-from gpueularsolver2d_amd.meshgen import (N_FAR, N_SURFACE, naca0012_mesh, naca0012_surface,
-                                          read_dat, write_dat)
+from gpueularsolver2d_amd.meshgen import (GRID, N_FAR, N_SURFACE, delaunay, naca0012_mesh,
+                                          naca0012_surface, read_dat, snap, write_dat)
 from oracle_ctypes import OracleMesh
 
 
@@ -62,15 +62,115 @@ def _digest(m):
 
 
 @pytest.mark.parametrize("name,cells,nodes,digest", [
-    ("C1", 3024, 1600, "2d90e0a46cb20242"),
-    ("C2", 50740, 25458, "4390848570830f45"),
-    ("C3", 198578, 99377, "32a4cd2e02b5b100"),
+    ("C1", 3024, 1600, "3931efa133687dfe"),
+    ("C2", 50740, 25458, "00f000f2b299acd9"),
+    ("C3", 198578, 99377, "efe6aca29cf7a4c5"),
 ])
 def test_benchmark_meshes_are_pinned(name, cells, nodes, digest):
     # the configurations' meshes, bit for bit: coordinates and triangles as
-    # generated here (the generator triangulates with scipy's Qhull, so a
-    # different Qhull or numpy build that changed a mesh shows up here)
+    # generated here (numpy's seeded generator places the points; a numpy
+    # build that changed them shows up here)
     from gpueularsolver2d_amd import workloads
     m = workloads.config_mesh(getattr(workloads, name))
     assert (m.ncell, m.nnode) == (cells, nodes)
     assert _digest(m) == digest
+
+
+# --- the triangulator (csrc/meshgen_delaunay.cpp) -------------------------
+
+def _ints(v):
+    return [int(round(float(a) * GRID)) for a in v]
+
+
+def _incircle(a, b, c, d):
+    """Exact sign of the incircle determinant on integer points."""
+    adx, ady, bdx, bdy = a[0] - d[0], a[1] - d[1], b[0] - d[0], b[1] - d[1]
+    cdx, cdy = c[0] - d[0], c[1] - d[1]
+    det = ((adx * adx + ady * ady) * (bdx * cdy - bdy * cdx)
+           + (bdx * bdx + bdy * bdy) * (cdx * ady - cdy * adx)
+           + (cdx * cdx + cdy * cdy) * (adx * bdy - ady * bdx))
+    return (det > 0) - (det < 0)
+
+
+def _check_delaunay(x, y, tri, strict=True):
+    """Every triangle counter-clockwise with an empty circumcircle (brute
+    force, exact integers); returns twice the covered area."""
+    X, Y = _ints(x), _ints(y)
+    P = list(zip(X, Y))
+    area2 = 0
+    for a, b, c in tri.tolist():
+        o = (P[b][0] - P[a][0]) * (P[c][1] - P[a][1]) - (P[b][1] - P[a][1]) * (P[c][0] - P[a][0])
+        assert o > 0, (a, b, c)
+        area2 += o
+        for d in range(len(P)):
+            if d not in (a, b, c):
+                s = _incircle(P[a], P[b], P[c], P[d])
+                assert s < 0 if strict else s <= 0, (a, b, c, d)
+    return area2
+
+
+def _ring_and_interior(n_ring, n_in, seed):
+    """Points on a circle (a convex hull whose edges have empty diametral
+    circles) and random interior points kept out of those circles."""
+    rng = np.random.default_rng(seed)
+    phi = 2 * np.pi * np.arange(n_ring) / n_ring
+    rx, ry = 3.0 * np.cos(phi), 3.0 * np.sin(phi)
+    r = 3.0 * np.sqrt(rng.uniform(0, 1, 4 * n_in))
+    t = rng.uniform(0, 2 * np.pi, 4 * n_in)
+    px, py = r * np.cos(t), r * np.sin(t)
+    mx, my = 0.5 * (rx + np.roll(rx, -1)), 0.5 * (ry + np.roll(ry, -1))
+    rad = 0.5 * np.hypot(np.roll(rx, -1) - rx, np.roll(ry, -1) - ry)
+    ok = np.ones(len(px), bool)
+    for i in range(n_ring):
+        ok &= np.hypot(px - mx[i], py - my[i]) > rad[i] * 1.01
+    return snap(np.concatenate([rx, px[ok][:n_in]])), snap(np.concatenate([ry, py[ok][:n_in]]))
+
+
+def _polygon_area2(x, y):
+    X, Y = _ints(x), _ints(y)
+    n = len(X)
+    return sum(X[i] * Y[(i + 1) % n] - X[(i + 1) % n] * Y[i] for i in range(n))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_triangulator_is_delaunay_and_covers_the_hull(seed):
+    x, y = _ring_and_interior(24, 120, seed)
+    tri = delaunay(x, y)
+    area2 = _check_delaunay(x, y, tri)
+    # exactly the circle polygon's area: no hole, no overlap
+    assert area2 == _polygon_area2(x[:24], y[:24])
+    # Euler: 2n - h - 2 triangles for n points, h of them on the hull
+    assert len(tri) == 2 * len(x) - 24 - 2
+
+
+def test_triangulator_on_cocircular_grid():
+    # every unit square of a grid has four cocircular corners: any diagonal
+    # is Delaunay, and the cover must still be exact
+    g = np.arange(12) * 0.25
+    X, Y = np.meshgrid(g, g)
+    x, y = X.ravel(), Y.ravel()
+    tri = delaunay(x, y)
+    assert len(tri) == 2 * 11 * 11
+    area2 = _check_delaunay(x, y, tri, strict=False)
+    assert area2 == 2 * _ints([2.75])[0] ** 2
+
+
+def test_triangulator_rejects_points_off_its_grid():
+    with pytest.raises(RuntimeError):
+        delaunay(np.array([0.0, 1.0, 0.3]), np.array([0.0, 0.0, 1.0 / 3.0]))
+    with pytest.raises(RuntimeError):
+        delaunay(np.array([0.0, 1.0, 0.0, 0.0]), np.array([0.0, 0.0, 1.0, 1.0]))
+
+
+def test_mesh_equals_scipy_delaunay_when_available():
+    # a cross-check only: the generator itself does not use scipy
+    spatial = pytest.importorskip("scipy.spatial")
+    from gpueularsolver2d_amd.meshgen import _inside_polygon
+    for target, seed in ((3000, 1), (20000, 5)):
+        m = naca0012_mesh(target, seed)
+        ref = spatial.Delaunay(np.column_stack([m.x, m.y])).simplices
+        cx, cy = m.x[ref].mean(axis=1), m.y[ref].mean(axis=1)
+        ref = ref[~_inside_polygon(cx, cy, m.x[:N_SURFACE], m.y[:N_SURFACE])]
+        ours = {tuple(t) for t in np.sort(m.tri - 1, axis=1).tolist()}
+        theirs = {tuple(t) for t in np.sort(ref, axis=1).tolist()}
+        assert ours == theirs
