@@ -85,16 +85,20 @@ def timed_run(solver, warmup, steps):
 
 
 def kernel_source_sha():
-    """Hash of the sources of the FP32 tiled kernels and their build flags: the
-    measured traffic in profiles/ applies only to the kernel it was measured on."""
+    """Hash of the sources of the FP32 tiled kernels and their compiler flags
+    (include paths left out, so the hash does not depend on where the
+    repository lies): the measured traffic in profiles/ applies only to the
+    kernel it was measured on."""
+    from gpueularsolver2d_amd import build as B
     h = hashlib.sha256()
     csrc = os.path.join(ROOT, "gpueularsolver2d_amd", "csrc")
     for f in ("fvm_tiled.h", "fvm_numerics.h", "fvm_internal.h", "fvm_launch.h",
               "fvm_tiled_inst.inc", "fvm_tiled32.hip"):
         with open(os.path.join(csrc, f), "rb") as fh:
             h.update(fh.read())
-    with open(os.path.join(ROOT, "gpueularsolver2d_amd", "build.py"), "rb") as fh:
-        h.update(fh.read())
+    flags = list(B.COMMON) + list(dict(B.SOURCES)["fvm_tiled32.hip"])
+    flags = [f for i, f in enumerate(flags) if f != "-I" and (i == 0 or flags[i - 1] != "-I")]
+    h.update(" ".join(flags).encode())
     return h.hexdigest()[:16]
 
 

@@ -50,10 +50,10 @@ def test_fp32_one_and_two_kernel_paths_agree_at_bench_size(bench_mesh):
 def test_fp32_tracks_fp64_at_bench_size(bench_mesh):
     # FP32 bulk against the FP64 path (= the oracle bit for bit) over the
     # first steps of the transient, before the limiter and smoothing dynamics
-    # amplify rounding differences (about 60-fold from step 10 to step 30
+    # amplify rounding differences (about 30-fold from step 10 to step 30
     # here).  Error of each conservative relative to the L2 norm of the whole
-    # state; measured on MI355X: max 1.9e-7 after 10 steps, 6.1e-6 after 30;
-    # dCl 3.1e-8 and 1.7e-5.
+    # state; measured on MI355X: max 1.5e-7 after 10 steps, 4.7e-6 after 30;
+    # dCl -6.8e-8 and -1.7e-6.
     tol = {10: dict(state=1e-6, force=1e-6), 30: dict(state=1e-4, force=1e-4)}
     out = {}
     for prec in ("fp32", "fp64"):

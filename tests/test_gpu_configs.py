@@ -62,8 +62,8 @@ def test_c2_fp32_tracks_oracle(c2):
     (cl, cd), (clo, cdo) = s.forces(), orc.forces()
     print(f"C2 FP32 vs oracle after 10 steps: state {err:.2e}, dCl {cl - clo:.2e}, "
           f"dCd {cd - cdo:.2e}", flush=True)
-    # measured on MI355X: 1.3e-6, dCl -7.8e-7, dCd -1.4e-8 (the coarser 50k
-    # mesh amplifies rounding more than the 200k one, 1.9e-7 there)
+    # measured on MI355X: 1.2e-6, dCl 7.6e-8, dCd 3.1e-7 (the coarser 50k
+    # mesh amplifies rounding more than the 200k one, 1.5e-7 there)
     assert err < 5e-6
     assert abs(cl - clo) < 5e-6 and abs(cd - cdo) < 5e-6
 
@@ -113,9 +113,10 @@ def test_c5_fp32_batch_tracks_fp64_batch(c5):
     # is smaller in magnitude (gpu_solver.cu:1714-1718), a choice that jumps by
     # |R| + |R2| when they have opposite signs and nearly equal magnitudes,
     # and R, a difference of O(1) fluxes, carries a relative FP32 error far
-    # above FP32's epsilon.  Case 185 (M 0.78) meets such a near-tie in the
-    # energy of one wall cell at step 10 (the oracle has R = -3.38682e-4,
-    # R2 = +3.38696e-4 there), which moves its Cl by 1.7e-4.
+    # above FP32's epsilon.  Such a near-tie in the energy of one wall cell at
+    # step 10 (R = -3.38682e-4, R2 = +3.38696e-4 in the oracle) once moved a
+    # case's Cl by 1.7e-4; on the current meshes the largest |dCl| is 9.4e-5
+    # (measured on MI355X).
     assert np.median(err) < 1e-6 and np.percentile(err, 95) < 5e-6
     assert np.median(dcl) < 1e-6 and np.percentile(dcl, 95) < 5e-6
     assert dcl.max() < 5e-4 and dcd.max() < 5e-4

@@ -117,14 +117,12 @@ def test_fp32_first_order_converged_vs_oracle(naca3k):
 
 def test_fp32_second_order_mean_forces(naca3k):
     # Second order with the per-face limiter settles into a small limit cycle
-    # (the residual stalls; Cl cycles with a std of 1.9e-4 here), so
-    # instantaneous FP32/FP64 states decorrelate; the time-averaged
-    # coefficients must agree to the |dCl| < 1e-4 bar.  The mean is taken
-    # every 10 steps over 4000 steps: sampling every 100 steps aliases with the
-    # cycle (20 such samples gave +1.5e-4 where 400 dense ones give -5.9e-5;
-    # FP64 from a 1e-12-perturbed start gives -1.8e-5 and dCd -1.4e-6, FP32
-    # dCd -1.5e-5, measured on MI355X).  The FP64 tiled path is the FP64 oracle
-    # bit for bit (test_fp64_bitwise_vs_oracle).
+    # (the residual stalls and Cl cycles), so instantaneous FP32/FP64 states
+    # decorrelate; the time-averaged coefficients must agree to the
+    # |dCl| < 1e-4 bar.  The mean is taken every 10 steps over 4000 steps:
+    # sampling every 100 steps would alias with the cycle (measured on MI355X:
+    # dCl +3.1e-5, dCd +1.2e-6).  The FP64 tiled path is the FP64 oracle bit
+    # for bit (test_fp64_bitwise_vs_oracle).
     s32 = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp32")
     s64 = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp64")
     s32.run(8000)
