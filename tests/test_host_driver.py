@@ -166,3 +166,53 @@ def test_driver_forces_match_integrated_wall_pressure(driver, tmp_path):
     assert cl > 0.1
     assert abs(clp - cl) < 0.06 * abs(cl)
     assert abs(cdp - cd) < 0.015
+
+
+def read_resid(d):
+    rows = [l.split() for l in open(os.path.join(d, "unresidual.plt")).read().splitlines()[3:]]
+    return {int(a): float(b) for a, b in rows}
+
+
+@pytest.mark.gpu
+def test_driver_global_dt_reproduces_reference_file(driver, tmp_path):
+    # the reference's unresidual_time2.plt (global time step, smoothing on),
+    # through the host program's --global-dt, to every printed digit
+    write_case(str(tmp_path), 2000)
+    r = subprocess.run([driver, "--global-dt", "--resid-every", "100", "--quiet"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = read_resid(tmp_path)
+    with open(os.path.join(GOLDEN, "reference_residuals.json")) as f:
+        ref = json.load(f)["unresidual_time2"]
+    assert len(ref) == 20
+    for stepno, val in ref:
+        assert abs(got[stepno] - val) < 0.6e-10, (stepno, got[stepno], val)
+
+
+@pytest.mark.gpu
+def test_driver_plain_kernels_write_the_same_files(driver, driver_run, tmp_path):
+    # the reference's kernel structure (--plain) gives the default kernels'
+    # FP64 bits: the same residual, node-average and wall-pressure files
+    write_case(str(tmp_path), 500)
+    r = subprocess.run([driver, "--no-smooth", "--plain", "--resid-every", "10", "--quiet"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for name in ("unresidual.plt", "unresult.plt", "unairfoilpress.plt"):
+        with open(os.path.join(driver_run, name)) as a, open(os.path.join(tmp_path, name)) as b:
+            assert a.read() == b.read(), name
+
+
+@pytest.mark.gpu
+def test_driver_fp32_tracks_fp64(driver, driver_run, tmp_path):
+    # --fp32: the same run in FP32 bulk; its residual history follows the
+    # FP64 one over these 500 transient steps (measured on MI355X: largest
+    # relative difference 3.0e-6)
+    write_case(str(tmp_path), 500)
+    r = subprocess.run([driver, "--no-smooth", "--fp32", "--resid-every", "10", "--quiet"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    a, b = read_resid(driver_run), read_resid(tmp_path)
+    assert sorted(a) == sorted(b) and len(a) == 50
+    rel = max(abs(b[k] - a[k]) / a[k] for k in a)
+    print(f"FP32 vs FP64 residual history, max relative difference {rel:.2e}", flush=True)
+    assert rel < 3e-5
