@@ -172,6 +172,54 @@ def cpu_baseline(mesh, steps, omp_steps):
                                    sample=f"{n1} time steps, gcc -O2, one thread, {s1:.1f} s"))
 
 
+# The reference's own 51,800-cell RAE 2822 mesh (2822-5w.dat), on which the
+# thesis timed its final CUDA code: 19.109 ms per step on a Tesla C2050, FP64
+# (10.84 Mcsu/s), and 811.75 ms per step for the one-thread CPU code on a
+# Phenom 9850.  The flow is the reference's printed case (M 0.729, alpha
+# 2.31 deg, CFL 1.0, second order with smoothing).
+REF_MESH_5W = os.path.join(ROOT, "tests", "golden", "2822-5w.dat.gz")
+REF_FLOW = dict(mach=0.729, alpha_deg=2.31, cfl=1.0)
+C2050_MS_PER_STEP = 19.109
+PHENOM_MS_PER_STEP = 811.75
+
+
+def reference_mesh(steps):
+    """The three GPU paths on the reference's 51,800-cell mesh, beside the
+    thesis's C2050 time for the same mesh (other hardware, FP64)."""
+    from gpueularsolver2d_amd import Solver, read_dat
+    mesh = read_dat(REF_MESH_5W)
+    C = mesh.ncell
+    out = dict(mesh="2822-5w.dat (the reference's)", cells=C, steps=steps, flow=REF_FLOW,
+               c2050_fp64=dict(ms_per_step=C2050_MS_PER_STEP,
+                               mcsu_per_s=round(4.0 * C / C2050_MS_PER_STEP / 1e3, 2),
+                               source="thesis Table 8; other hardware"))
+    for name, kw in (("fp32", dict(precision="fp32")), ("fp64", dict(precision="fp64")),
+                     ("fp64_plain", dict(precision="fp64", kernels="plain"))):
+        s = Solver(mesh, **REF_FLOW, **kw)
+        t, _ = timed_run(s, 10, steps)
+        out[name] = dict(value=round(4.0 * C * steps / t / 1e6, 1), unit="Mcsu/s",
+                         ms_per_step=round(1e3 * t / steps, 4),
+                         x_c2050=round(C2050_MS_PER_STEP / (1e3 * t / steps), 1))
+        del s
+    return out
+
+
+def cpu_reference_mesh(steps):
+    """The one-thread oracle on the reference's 51,800-cell mesh, beside the
+    thesis's one-thread CPU time for it (part of the CPU baseline)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from gpueularsolver2d_amd import read_dat
+    from oracle_ctypes import LIB_PATH, OracleMesh, OracleSolver
+    mesh = read_dat(REF_MESH_5W)
+    om = OracleMesh(mesh.x, mesh.y, mesh.tri, library=LIB_PATH)
+    s = OracleSolver(om, REF_FLOW["mach"], REF_FLOW["alpha_deg"], cfl=REF_FLOW["cfl"])
+    s.time_steps(1)
+    sec = s.time_steps(steps)
+    return dict(cells=mesh.ncell, steps=steps, ms_per_step=round(1e3 * sec / steps, 2),
+                value=round(4.0 * mesh.ncell * steps / sec / 1e6, 2), unit="Mcsu/s", cores=1,
+                phenom9850_ms_per_step=PHENOM_MS_PER_STEP)
+
+
 def perturbed(W, rng):
     return W * (1.0 + 1e-12 * rng.standard_normal(W.shape))
 
@@ -444,6 +492,8 @@ def main():
                     help="steps of the 256-case batch (0 skips it)")
     ap.add_argument("--window", type=int, default=500,
                     help="steps of the per-case accuracy window of C2 and the batches")
+    ap.add_argument("--refmesh-steps", type=int, default=1000,
+                    help="steps of the GPU paths on the reference's 51,800-cell mesh (0 skips it)")
     ap.add_argument("--cpu-steps", type=int, default=100,
                     help="steps of the one-thread CPU sample")
     ap.add_argument("--cpu-omp-steps", type=int, default=200,
@@ -455,6 +505,9 @@ def main():
                     help="time only the FP32 path (for profiler runs)")
     ap.add_argument("--fp64-only", action="store_true",
                     help="time only the FP64 one-kernel-per-stage path (for profiler runs)")
+    ap.add_argument("--mesh", choices=["c3", "2822-5w"], default="c3",
+                    help="mesh of the --fp32-only, --fp64-only and --plain-only modes: the "
+                         "bench mesh, or the reference's 51,800-cell mesh at its printed case")
     ap.add_argument("--plain-only", action="store_true",
                     help="time only the FP64 plain path, the reference's kernel structure "
                          "(for profiler runs)")
@@ -489,11 +542,17 @@ def main():
         print(json.dumps({"copy_mib": args.copy_mb}))
         return 0
 
-    from gpueularsolver2d_amd import Solver, workloads
+    from gpueularsolver2d_amd import Solver, read_dat, workloads
     cfg = workloads.C3
     mesh = workloads.config_mesh(cfg)
-    C = mesh.ncell
     flow = cfg["flow"]
+    if args.mesh == "2822-5w":
+        if not (args.plain_only or args.fp32_only or args.fp64_only):
+            print("--mesh 2822-5w applies to the --fp32-only, --fp64-only and --plain-only modes",
+                  file=sys.stderr)
+            return 2
+        mesh, flow = read_dat(REF_MESH_5W), REF_FLOW
+    C = mesh.ncell
     csu = 4.0 * C * args.steps
 
     if args.plain_only:
@@ -561,9 +620,13 @@ def main():
     progress("64-case batch done")
     b256 = batch256(args.batch256_steps, window=args.window) if args.batch256_steps > 0 else None
     progress("256-case batch done")
+    refm = reference_mesh(args.refmesh_steps) if args.refmesh_steps > 0 else None
+    progress("reference mesh done")
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         cpu = cpu_baseline(mesh, args.cpu_steps, args.cpu_omp_steps)
+        if args.refmesh_steps > 0:
+            cpu["reference_mesh_2822_5w"] = cpu_reference_mesh(20)
         progress("cpu baseline done")
 
     out = {
@@ -577,6 +640,7 @@ def main():
                    "kernels": "tiled", "launch": s32.config()},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "reference_mesh_2822_5w": refm,
         "fp64_tiled": {"value": round(v64, 1), "kernel_ms": ms64_region / (4 * args.steps),
                        "kernel_ms_isolated": kt64[0]},
         "fp64_plain": {"value": round(vpl, 1), "steps": plain_steps},

@@ -5,8 +5,9 @@ reference_residuals.json
     unresidual_nosmooth.plt, unresidual_smooth.plt, unresidual_time2.plt,
     3558-cell RAE 2822 mesh 2822-3k.dat), truncated to their first 500 steps.
     Data only: (step, Resd) pairs.
-2822-3k.dat.gz, 2822-7k.dat.gz
-    The reference's own mesh files, gzip-compressed.
+2822-3k.dat.gz, 2822-7k.dat.gz, 2822-5w.dat.gz
+    The reference's own mesh files (3558, 13513 and 51800 cells), gzip-compressed.
+    The 51800-cell mesh is the one the thesis's per-kernel timings were taken on.
 oracle_vectors.npz
     Regression vectors of the FP64 oracle on small cases (residual history,
     final state, Cl/Cd): they pin the oracle against unintended change, not
@@ -46,9 +47,10 @@ def reference_fixtures():
         d[name] = read_plt(os.path.join(REF, name + ".plt"), 500 if "time2" not in name else 2000)
     with open(os.path.join(HERE, "reference_residuals.json"), "w") as f:
         json.dump(d, f, indent=0)
-    for m in ("2822-3k", "2822-7k"):
+    for m in ("2822-3k", "2822-7k", "2822-5w"):
         with open(os.path.join(REF, m + ".dat"), "rb") as src, \
-                gzip.open(os.path.join(HERE, m + ".dat.gz"), "wb", compresslevel=9) as dst:
+                open(os.path.join(HERE, m + ".dat.gz"), "wb") as raw, \
+                gzip.GzipFile(fileobj=raw, mode="wb", compresslevel=9, mtime=0) as dst:
             shutil.copyfileobj(src, dst)
 
 

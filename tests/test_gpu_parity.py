@@ -176,6 +176,30 @@ def test_fp64_bitwise_on_reference_7k_mesh():
         np.testing.assert_allclose(hist[1:], hist_o[1:steps + 1], rtol=1e-12, atol=0)
 
 
+def test_reference_5w_mesh_bitwise_and_fp32():
+    # the reference's 51,800-cell mesh (2822-5w, the one the thesis timed its
+    # kernels on): the default launch shape and the plain kernels bit for bit
+    # against the oracle over 20 steps of the RAE 2822 case, and FP32 beside
+    # them (measured on MI355X: density relative L2 error 9.6e-6 after 20 steps)
+    mesh = read_dat(os.path.join(GOLDEN, "2822-5w.dat.gz"))
+    assert mesh.ncell == 51800
+    steps = 20
+    orc, hist_o = oracle_run(mesh, steps, 0.729, 2.31, 1.0)
+    Wo = orc.state()
+    for kernels in ("tiled", "plain"):
+        s = Solver(mesh, 0.729, 2.31, cfl=1.0, precision="fp64", kernels=kernels)
+        hist, n, st = s.run(steps)
+        assert st == 0 and n == steps
+        assert np.array_equal(s.state(), Wo), kernels
+        np.testing.assert_allclose(hist[1:], hist_o[1:steps + 1], rtol=1e-12, atol=0)
+    s32 = Solver(mesh, 0.729, 2.31, cfl=1.0, precision="fp32")
+    _, n, st = s32.run(steps)
+    assert st == 0 and n == steps
+    err = rel_l2(s32.state()[:, 0], Wo[:, 0])
+    print(f"2822-5w FP32 vs oracle after {steps} steps: density rel. L2 {err:.2e}", flush=True)
+    assert err < 5e-5
+
+
 def supersonic_far_edges(mesh, mach, alpha, t_in):
     """Far-field edges on which the reference takes the supersonic branch
     (gpu_solver.cu:79: |u_inf . n| / AIN > 1 with the dimensional AIN) at the
