@@ -280,9 +280,11 @@ class Batch {
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   hipGraphExec_t graph_[2] = {nullptr, nullptr};  // two instances of the same steps, alternated
-  // graphs of 2^k steps below steps_per_graph_ (largest first), for the steps
-  // of a run that do not fill a whole graph
+  // graphs of every even step count below steps_per_graph_ (at most
+  // kLadderSlots, smallest first), for the steps of a run that do not fill a
+  // whole graph
   std::vector<std::pair<int, hipGraphExec_t>> ladder_;
+  int split_rest(int rest, int* chunks, int max_chunks) const;
   std::vector<int> before_;          // per case: step count at the start of a run
   int n_partial_ = 0;                // per-tile (or per-block) partial sums of the step close
   std::vector<Topology> topo_;
@@ -430,6 +432,8 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
     setup_tiled<float>();
   // 50 steps per graph: 20 measured 1% slower on the 198k-cell case (a
   // trailing close launch and a host poll per graph)
+  if (o.steps_per_graph > 1024)
+    throw Error{FVM_ERR_ARGUMENT, "opts.steps_per_graph must be at most 1024"};
   steps_per_graph_ = o.steps_per_graph > 0 ? o.steps_per_graph : 50;
   // the one-kernel-per-stage graph alternates the step-start buffers, so it
   // holds an even number of steps
@@ -440,6 +444,9 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
   // a residual history long enough for runs of up to kInitialHistory steps,
   // so that such runs allocate nothing
   ensure_history((int64_t)kInitialHistory + 1);
+  // the graphs are captured, instantiated and uploaded here (about 5 ms of
+  // host work) rather than at the first run, which then starts at once
+  build_graph();
   HIP_OK(hipStreamSynchronize(stream_));
 }
 
@@ -1113,9 +1120,11 @@ void Batch::drop_graphs() {
 
 // Two executable instances of the same captured steps (a replay can be queued
 // while the previous one, of the other instance, still runs), and one graph
-// of each power-of-two step count below it.  A run of any length is then
-// whole graphs and at most one directly launched step: every graph holds an
-// even number of steps, as the one-kernel-per-stage path needs (its
+// of each even step count below it.  A run of any length is then whole
+// graphs and at most one directly launched step, and a run of up to
+// steps_per_graph_ steps is one graph (each graph ends with a close launch, and
+// each chunk boundary can leave the device waiting on the host): every graph
+// holds an even number of steps, as the one-kernel-per-stage path needs (its
 // step-start buffer alternates by the step's parity within the graph).
 void Batch::build_graph() {
   drop_graphs();
@@ -1139,10 +1148,25 @@ void Batch::build_graph() {
   };
   graph_[0] = capture(steps_per_graph_, 0);
   graph_[1] = capture(steps_per_graph_, 1);
-  int k = 2;
-  while (k * 2 < steps_per_graph_) k *= 2;
-  for (; k >= 2 && (int)ladder_.size() < kLadderSlots; k /= 2)
+  for (int k = 2; k < steps_per_graph_ && (int)ladder_.size() < kLadderSlots; k += 2)
     ladder_.emplace_back(k, capture(k, 2 + (int)ladder_.size()));
+}
+
+// The steps of a run that do not fill a whole graph, as ladder graphs
+// (greedily the largest that fits, then listed smallest first) and at most
+// one single step, last; returns the number of chunks.
+int Batch::split_rest(int rest, int* chunks, int max_chunks) const {
+  int n = 0;
+  for (int r = rest & ~1; r >= 2 && n < max_chunks - 1;) {
+    int k = 2;
+    for (const auto& l : ladder_)
+      if (l.first <= r) k = std::max(k, l.first);
+    chunks[n++] = k;
+    r -= k;
+  }
+  std::reverse(chunks, chunks + n);
+  if (rest & 1) chunks[n++] = 1;
+  return n;
 }
 
 // Residual history of at least `rows` entries per case, in pinned host
@@ -1209,13 +1233,12 @@ void Batch::run_steps(int max_steps, double epsl, Launch launch) {
     const int remaining = max_steps - m0;
     const int nfull = remaining / steps_per_graph_;
     const int rest = remaining % steps_per_graph_;
-    // the 2^k chunks first, smallest first: the host's work to launch the
-    // next, larger graph then overlaps the device's work on a small one (a
-    // run's first graph launch is otherwise exposed); the single odd step
-    // last, after an even number of steps
-    int ladder[32], nl = 0;
-    for (int k = 2; k <= rest && nl < 32; k *= 2)
-      if (rest & k) ladder[nl++] = k;
+    // the ladder chunks first, smallest first: the host's work to launch the
+    // next, larger graph then overlaps the device's work on a small one; the
+    // single odd step last, after an even number of steps
+    int ladder[64];
+    const int nsplit = split_rest(rest, ladder, 64);
+    const int nl = nsplit - (rest & 1);
     const int nchunks = nl + nfull + (rest & 1);
     bool any = true;
     CaseCtl* prev = nullptr;

@@ -86,7 +86,7 @@ typedef struct fvm_scheme_opts {
     int32_t kernels;        /* fvm_kernels */
     int32_t tile_cells;     /* cells per workgroup tile, 0 = chosen by the library */
     int32_t threads;        /* threads per workgroup, 0 = 512 (one kernel per stage) or 256 */
-    int32_t steps_per_graph;/* time steps per captured hipGraph, 0 = automatic */
+    int32_t steps_per_graph;/* time steps per captured hipGraph, 0 = automatic (50), at most 1024 */
     int32_t device;         /* HIP device ordinal */
 } fvm_scheme_opts;
 

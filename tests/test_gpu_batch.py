@@ -75,9 +75,9 @@ def test_resume_equals_one_run(meshes):
 
 
 def test_run_lengths_cover_graph_ladder(meshes):
-    # a run is whole graphs of steps_per_graph steps, graphs of 2^k steps and
-    # at most one direct step; any split of a run into calls gives the same
-    # bits as one call
+    # a run is whole graphs of steps_per_graph steps, one graph of each even
+    # step count below it and at most one direct step; any split of a run
+    # into calls gives the same bits as one call
     for precision, kernels in (("fp32", "tiled"), ("fp64", "tiled"), ("fp64", "split"),
                                ("fp64", "plain")):
         a = Solver(meshes[0], precision=precision, kernels=kernels, **FLOWS[2])
@@ -187,3 +187,23 @@ def test_batch_set_state_restores_a_case(meshes):
         assert np.array_equal(a.state(2), b.state(2))
         with pytest.raises(FvmError):
             b.set_state(3, W)
+
+
+def test_long_graphs_split_runs_into_ladder_chunks(meshes):
+    # with 200 steps per graph the even ladder stops at 64 steps (32 graphs),
+    # so a run of 199 steps is chunks of 6, 64, 64, 64 and one direct step;
+    # the bits equal those of the default 50-step graphs
+    a = Solver(meshes[0], precision="fp32", steps_per_graph=200, **FLOWS[2])
+    assert a.config()["steps_per_graph"] == 200
+    ha, n, st = a.run(199)
+    assert n == 199 and st == 0
+    hb2, n, st = a.run(401)   # one whole graph and a 2-step chunk
+    assert n == 401 and st == 0
+    b = Solver(meshes[0], precision="fp32", **FLOWS[2])
+    hb, _, _ = b.run(401)
+    assert np.array_equal(a.state(), b.state())
+    np.testing.assert_array_equal(ha[1:], hb[1:200])
+    np.testing.assert_array_equal(hb2[200:], hb[200:])
+    with pytest.raises(FvmError) as e:
+        Solver(meshes[0], precision="fp32", steps_per_graph=1025, **FLOWS[2])
+    assert "steps_per_graph" in str(e.value)
