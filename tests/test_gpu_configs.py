@@ -4,6 +4,9 @@
   order with smoothing: the plain FP64 kernels (the reference's structure) and
   the tiled FP64 kernels bit for bit against the oracle, and the FP32 path
   against the oracle over the first steps of the transient;
+* C4, the 64-case batch (four meshes of ~3k-90k triangles, 16 cases each):
+  the FP64 batch bit for bit against the oracle on one case of each mesh, and
+  the FP32 batch against the FP64 batch case by case;
 * C5, the 256-case batch of uneven meshes (2k-150k triangles, ~9.5M cells):
   the FP64 batch bit for bit against the oracle on its smallest, median and
   largest case, and the FP32 batch against the FP64 batch case by case.
@@ -66,6 +69,47 @@ def test_c2_fp32_tracks_oracle(c2):
     # mesh amplifies rounding more than the 200k one, 1.5e-7 there)
     assert err < 5e-6
     assert abs(cl - clo) < 5e-6 and abs(cd - cdo) < 5e-6
+
+
+@pytest.fixture(scope="module")
+def c4():
+    return workloads.batch64()
+
+
+def test_c4_fp64_batch_bitwise_vs_oracle(c4):
+    meshes, case_mesh, flows = c4
+    steps = 5
+    b = Batch(meshes, case_mesh, flows, precision="fp64")
+    print(f"C4 FP64 batch: 64 cases on meshes of {[m.ncell for m in meshes]} cells, "
+          f"launch {b.config()}", flush=True)
+    _, n, st = b.run(steps)
+    assert list(n) == [steps] * 64 and not st.any()
+    # one case per mesh, each at a different (Mach, alpha) pair
+    for i in (0, 21, 42, 63):
+        orc, _ = oracle(meshes[case_mesh[i]], flows[i], steps)
+        assert np.array_equal(b.state(i), orc.state()), f"case {i}"
+
+
+def test_c4_fp32_batch_tracks_fp64_batch(c4):
+    meshes, case_mesh, flows = c4
+    steps = 10
+    out = {}
+    for prec in ("fp32", "fp64"):
+        b = Batch(meshes, case_mesh, flows, precision=prec)
+        _, n, st = b.run(steps)
+        assert list(n) == [steps] * 64 and not st.any()
+        out[prec] = ([b.state(i) for i in range(64)], b.forces())
+        del b
+    W32, (cl32, cd32) = out["fp32"]
+    W64, (cl64, cd64) = out["fp64"]
+    err = np.array([state_err(a, b) for a, b in zip(W32, W64)])
+    dcl, dcd = np.abs(cl32 - cl64), np.abs(cd32 - cd64)
+    print(f"C4 FP32 vs FP64 after {steps} steps: state max {err.max():.2e} median "
+          f"{np.median(err):.2e}; |dCl| max {dcl.max():.2e}; |dCd| max {dcd.max():.2e}", flush=True)
+    # the same bars as the 256-case batch below (and the reasons for them)
+    assert np.median(err) < 1e-6 and np.percentile(err, 95) < 5e-6
+    assert np.median(dcl) < 1e-6 and np.percentile(dcl, 95) < 5e-6
+    assert dcl.max() < 5e-4 and dcd.max() < 5e-4
 
 
 @pytest.fixture(scope="module")
