@@ -390,8 +390,20 @@ __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc
                                            const R* sB, SVec4<R>* sF, R* sS, int ML,
                                            const EdgeRec<R>& e0) {
   const int s0 = threadIdx.x;
+  // FP32: the record of this thread's next edge is requested before this
+  // edge is computed, so an edge beyond the first blockDim costs no round
+  // trip of its own (FP64's 80-byte record would hold 20 more registers)
+  constexpr bool kNext = sizeof(R) == 4;
+  EdgeRec<R> en{};
+  if (kNext && kPre && s0 < d.nedge) en = e0;
   for (int s = threadIdx.x; s < d.nedge; s += blockDim.x) {
-    const EdgeRec<R> e = (kPre && s == s0) ? e0 : a.edges[d.edge0 + s];
+    EdgeRec<R> e;
+    if constexpr (kNext) {
+      e = (!kPre && s == s0) ? a.edges[d.edge0 + s] : en;
+      if (s + (int)blockDim.x < d.nedge) en = a.edges[d.edge0 + s + blockDim.x];
+    } else {
+      e = (kPre && s == s0) ? e0 : a.edges[d.edge0 + s];
+    }
     const EdgeView ev = edge_cells(e);
     const R vx = edge_vx(e), vy = edge_vy(e);
     const Vec4<R> WL = ld(sW[ev.l]);
@@ -758,7 +770,36 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
       sR[M4] = SVec4<R>{R(-0.0), R(-0.0), R(-0.0), R(-0.0)};
       sR1[MR] = SVec4<R>{R(-0.0), R(-0.0), R(-0.0), R(-0.0)};
     }
-    for (int j = threadIdx.x; j < n1; j += blockDim.x) {
+    // A thread's entries s0 and s0 + blockDim are staged together: both ring
+    // ids are requested, then both residuals, so that the entries beyond the
+    // first blockDim (rings 3-4 of larger tiles) cost no round trips of their
+    // own.  Entries beyond 2 blockDim (very large tiles) follow one by one.
+    {
+      const int j = s0, j2 = s0 + (int)blockDim.x;
+      // an entry past the end reads the tile's first cell (in bounds, unused),
+      // so that both residual loads are unconditional and in flight together
+      int rid = d.cell0, rid2 = d.cell0;
+      if (j < n1 && j >= nt) rid = a.rings[d.ring0 + j - nt];
+      if (j2 < n1 && j2 >= nt) rid2 = a.rings[d.ring0 + j2 - nt];
+      const int g = j < nt ? d.cell0 + j : rid;
+      const int g2 = j2 < nt ? d.cell0 + j2 : rid2;
+      const SVec4<R> r = Rin[g];
+      const SVec4<R> r2 = Rin[g2];
+      // both loads are issued here, before the stores that use them (left to
+      // itself the compiler moves each load into its store's branch, where
+      // the second waits for the first)
+      __asm__ volatile("" : : "v"(r.d), "v"(r.du), "v"(r.dv), "v"(r.de), "v"(r2.d), "v"(r2.du),
+                       "v"(r2.dv), "v"(r2.de));
+      if (j < n1) {
+        sR[j] = r;
+        if (j < NW) sGid[j] = g;
+      }
+      if (j2 < n1) {
+        sR[j2] = r2;
+        if (j2 < NW) sGid[j2] = g2;
+      }
+    }
+    for (int j = s0 + 2 * (int)blockDim.x; j < n1; j += blockDim.x) {
       const int g = gid(j);
       sR[j] = Rin[g];
       if (j < NW) sGid[j] = g;
@@ -768,12 +809,7 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     if (!s_active) return;
     // 2. smoothing sweep 1 on the tile and rings 1..3
     if (a.smoothing) {
-      for (int j = threadIdx.x; j < NR; j += blockDim.x) {
-        int2 lp;
-        if (j == s0)
-          lp = lp0;
-        else
-          lp = lnb[j];
+      auto sweep1 = [&](int j, int2 lp) {
         const int4 ln = unpack_tri16(lp);
         const int lk[3] = {ln.x, ln.y, ln.z};
         const SVec4<R> ri = sR[j];
@@ -788,7 +824,17 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
         const R den = R(1.0) + eps * R(ln.w);  // ln.w: CLog
         sR1[j] = SVec4<R>{(q0 * eps + ri.d) / den, (q1 * eps + ri.du) / den,
                           (q2 * eps + ri.dv) / den, (q3 * eps + ri.de) / den};
+      };
+      // the neighbour entry of s0 + blockDim is requested before entry s0
+      // (whose entry was read at kernel start) is computed
+      {
+        const int j2 = s0 + (int)blockDim.x;
+        int2 lp2 = make_int2(0, 0);
+        if (j2 < NR) lp2 = lnb[j2];
+        if (s0 < NR) sweep1(s0, lp0);
+        if (j2 < NR) sweep1(j2, lp2);
       }
+      for (int j = s0 + 2 * (int)blockDim.x; j < NR; j += blockDim.x) sweep1(j, lnb[j]);
       __syncthreads();
     }
     // 3. sweep 2, selection and the RK update on the tile and rings 1..2
