@@ -622,8 +622,8 @@ def main():
     v64 = csu / t64 / 1e6
     vpl = 4.0 * C * plain_steps / tpl / 1e6
     progress(f"fp64 tiled {v64:.1f}, fp64 plain {vpl:.1f} Mcsu/s")
-    # the global time step (kernel_TimeStep's mode): two kernels per stage with
-    # the per-case minimum between them
+    # the global time step (kernel_TimeStep's mode): one kernel per stage, each
+    # launch taking the case's minimum of the previous stage's tile minima
     gdt = {}
     gsteps = max(10, args.steps // 4)
     for prec in ("fp32", "fp64"):
@@ -631,7 +631,7 @@ def main():
         tg, _ = timed_run(sg, 5, gsteps, warm=Solver(mesh, precision=prec, global_dt=True, **flow))
         kt_g = sg.kernel_times(reps=10)
         gdt[prec] = dict(value=round(4.0 * C * gsteps / tg / 1e6, 1), steps=gsteps,
-                         flux_tile_ms=kt_g[0], update_tile_ms=kt_g[1])
+                         stage_tile_ms=kt_g[0])
 
     # the stage kernel's average launch time over the timed steps: the device
     # time of the steps (HIP events on the solver's stream) over their launches

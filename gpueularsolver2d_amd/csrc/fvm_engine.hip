@@ -325,6 +325,7 @@ class Batch {
   TileDesc* d_tiles_ = nullptr;
   void *d_w0_ = nullptr, *d_wc_ = nullptr, *d_res_ = nullptr, *d_dt_ = nullptr;
   void *d_tile_min_ = nullptr, *d_dtg_ = nullptr;
+  void* d_tile_min_b_ = nullptr;  // one kernel per stage: the tile minima alternate by stage
   // forces tables (tiled)
   WallTable walls_{};
   int *d_case_st0_ = nullptr, *d_case_w0_ = nullptr, *d_case_nw_ = nullptr;
@@ -404,7 +405,7 @@ Batch::Batch(int nmesh, const fvm_mesh_desc* meshes, int ncase, const int* case_
   // set-up of fvm_run_noalloc's scratch on this device (once per device)
   (void)scratch_on(device_);
   HIP_OK(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, device_));
-  fused_ = o.kernels == FVM_KERNELS_TILED && !o.global_dt;
+  fused_ = o.kernels == FVM_KERNELS_TILED;
   if (o.kernels != FVM_KERNELS_PLAIN) {
     if (o.tile_cells > 8192)
       throw Error{FVM_ERR_ARGUMENT, "opts.tile_cells must be at most 8192"};
@@ -649,6 +650,7 @@ void Batch::setup_tiled() {
       d_dt_b_ = arena_.take<R>(ncell_state);
       d_rings_ = arena_.take<int>(nring_all);
       d_lnb_ = arena_.take<int2>(nlnb_all);
+      d_tile_min_b_ = arena_.take<R>(tiles.size());
     }
     d_partial_ = arena_.take<double>(tiles.size());
     d_hdesc_ = arena_.take<HistDesc>(1);
@@ -973,6 +975,8 @@ void Batch::enqueue_fused_steps(int nsteps, RunBufs rb) {
   SVec4<R>* Rb = static_cast<SVec4<R>*>(d_res_b_);
   R* Da = static_cast<R*>(d_dt_);
   R* Db = static_cast<R*>(d_dt_b_);
+  R* Ma = static_cast<R*>(d_tile_min_);
+  R* Mb = static_cast<R*>(d_tile_min_b_);
   a.apply_update = 1;
   for (int s = 0; s < nsteps; s++) {
     a.parity = s & 1;
@@ -980,8 +984,10 @@ void Batch::enqueue_fused_steps(int nsteps, RunBufs rb) {
       const bool from_a = (k & 1) == 1;  // stages 1 and 3 read (a)
       a.r_in = from_a ? Ra : Rb;
       a.dt_in = from_a ? Da : Db;
+      a.tile_min_in = from_a ? Ma : Mb;
       a.res = from_a ? Rb : Ra;
       a.dt = from_a ? Db : Da;
+      a.tile_min = from_a ? Mb : Ma;
       a.rk = (R)kRK[k - 1];
       a.close_step = k == 4;
       a.close_prev = k == 1 && s > 0;
@@ -999,6 +1005,7 @@ void Batch::fused_init() {
   a.close_step = 0;
   a.res = static_cast<SVec4<R>*>(d_res_);
   a.dt = static_cast<R*>(d_dt_);
+  a.tile_min = static_cast<R*>(d_tile_min_);
   HIP_OK(launch_stage_tile<R>(a, threads_, lds_s_, stream_));
 }
 
@@ -1045,6 +1052,8 @@ void Batch::time_tiled(int reps, double ms[2]) {
   if (fused_) {
     a.res = static_cast<SVec4<R>*>(d_res_b_);
     a.dt = static_cast<R*>(d_dt_b_);
+    a.tile_min_in = static_cast<R*>(d_tile_min_);
+    a.tile_min = static_cast<R*>(d_tile_min_b_);
   }
   // every case active for the measurement, then restore the control block
   std::vector<CaseCtl> on = h_ctl_;

@@ -159,8 +159,10 @@ struct TiledArgs {
   SVec4<R>* res;              // residual written by kernel A / stage_tile
   R* dt;                      // local dt written by kernel A / stage_tile (FP32: dt / A; A*CFL/T in
                               // global mode)
-  R* tile_min;                // per tile minimum of A*CFL/T (global dt mode)
-  const R* dt_global;         // per case (global dt mode)
+  R* tile_min;                // per tile minimum of A*CFL/T (global dt mode); stage_tile: this
+                              // stage's, written
+  const R* tile_min_in;       // stage_tile, global dt mode: the previous stage's tile minima
+  const R* dt_global;         // per case (global dt mode, two-kernel path)
   const SVec4<R>* w0;         // step-start state (two-kernel path)
   SVec4<R>* w_out;            // kernel B output
   double* partial;            // per tile sum of (rho - rho0)^2
@@ -452,11 +454,11 @@ __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc
 
 // Ordered gather of each tile cell's three edge slots, R and the local time
 // step (kernel_Flux_e2c gpu_solver.cu:768-878).  Returns this thread's
-// minimum of A*CFL/T (global time step mode).  The local time step is stored
-// as dt in FP64 and as dt / A in FP32 (the RK update's only use of dt is
-// dt R / A, so FP32 folds the division by the area in here and the update
-// reads no area).  ar0: this thread's first cell's (1 / A, A), read at kernel
-// start.
+// minimum of A*CFL/T (global time step mode, which stores no per-cell time
+// step).  The local time step is stored as dt in FP64 and as dt / A in FP32
+// (the RK update's only use of dt is dt R / A, so FP32 folds the division by
+// the area in here and the update reads no area).  ar0: this thread's first
+// cell's (1 / A, A), read at kernel start.
 template <typename R>
 __device__ __forceinline__ R gather_cells(const TiledArgs<R>& a, const TileDesc& d, const FlowT<R>& f,
                                           const SVec4<R>* sF, const R* sS, SVec4<R>* Rout, R* dtout,
@@ -482,8 +484,8 @@ __device__ __forceinline__ R gather_cells(const TiledArgs<R>& a, const TileDesc&
     Rout[d.st0 + g] = SVec4<R>{r0, r1, r2, r3};
     const Pair<R> ar = (i == s0) ? ar0 : area_pair(a.cg + d.geo0, g);
     if (a.global_dt) {
+      // only the minimum over the case is used (kernel_TimeStep)
       const R v = ar.b * f.cfl / T;
-      dtout[d.st0 + g] = v;
       tmin = fmin(tmin, v);
     } else {
       const R dt = fmin(ar.b * f.cfl / T, R(0.1));
@@ -707,6 +709,7 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   // do not wait for the control block; the host stops launching once the
   // case stops.  With several, a stopped case's tiles leave before any load.
   __shared__ int s_active;
+  __shared__ R s_wmin[16];  // global time step: the waves' minima
   if (!a.apply_update) {
     if (cc.status != FVM_OK) return;  // no launch writes it meanwhile
   } else if (a.early_exit) {
@@ -807,6 +810,13 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     if (!a.early_exit && threadIdx.x == 0) s_active = cc.active;
     __syncthreads();
     if (!s_active) return;
+    R gm0 = R(9999999.0), gm1 = R(9999999.0);  // the reference's initial minimum
+    if (a.global_dt) {
+      const int n = cc.ntile, t0 = cc.tile0, k2 = s0 + (int)blockDim.x;
+      if (s0 < n) gm0 = a.tile_min_in[t0 + s0];
+      if (k2 < n) gm1 = a.tile_min_in[t0 + k2];
+      for (int k = k2 + (int)blockDim.x; k < n; k += blockDim.x) gm1 = fmin(gm1, a.tile_min_in[t0 + k]);
+    }
     // 2. smoothing sweep 1 on the tile and rings 1..3
     if (a.smoothing) {
       auto sweep1 = [&](int j, int2 lp) {
@@ -835,12 +845,35 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
         if (j2 < NR) sweep1(j2, lp2);
       }
       for (int j = s0 + 2 * (int)blockDim.x; j < NR; j += blockDim.x) sweep1(j, lnb[j]);
-      __syncthreads();
+    }
+    // Global time step (kernel_TimeStep, gpu_solver.cu:880-936): the
+    // previous stage's minimum of A*CFL/T over the case, from its tiles'
+    // minima (requested above, combined here: wave shuffles, then the waves'
+    // minima through LDS after the barrier).  A minimum does not depend on
+    // the order it is taken in, so every tile of the case gets the same value.
+    if (a.global_dt) {
+      R m = fmin(gm0, gm1);
+      for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+      if ((threadIdx.x & 63) == 0) s_wmin[threadIdx.x >> 6] = m;
+    }
+    if (a.smoothing || a.global_dt) __syncthreads();
+    R dtg = R(0);
+    if (a.global_dt) {
+      dtg = s_wmin[0];
+      for (int k = 1; k < (int)(blockDim.x >> 6); k++) dtg = fmin(dtg, s_wmin[k]);
     }
     // 3. sweep 2, selection and the RK update on the tile and rings 1..2
     for (int j = threadIdx.x; j < NW; j += blockDim.x) {
       const int g = sGid[j];
-      const R dtv = a.dt_in[d.st0 + g];  // FP32: dt / A
+      // FP32 local time step: dt / A; global: the case's dt, and FP32 then
+      // multiplies by 1 / A
+      R dtv;
+      if (a.global_dt) {
+        dtv = dtg;
+        if constexpr (sizeof(R) == 4) dtv *= (j == s0 && j < nt) ? ar0.a : area_pair(cg, g).a;
+      } else {
+        dtv = a.dt_in[d.st0 + g];
+      }
       R area = R(1);
       if constexpr (sizeof(R) == 8) {
         if (j == s0 && j < nt)
@@ -926,8 +959,13 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   // 5. flux and spectral radius of the tile's edges
   flux_edges<R, kPreEdge>(a, d, f, sW, sG, sB, sF, sS, ML, e0);
   __syncthreads();
-  // 6. ordered gather: this stage's residual and time step
-  gather_cells<R>(a, d, f, sF, sS, a.res, a.dt, sl0, ar0);
+  // 6. ordered gather: this stage's residual and time step (global time
+  //    step: the tile's minimum of A*CFL/T)
+  const R tmin = gather_cells<R>(a, d, f, sF, sS, a.res, a.dt, sl0, ar0);
+  if (a.global_dt) {
+    const R m = block_min(tmin);
+    if (threadIdx.x == 0) a.tile_min[t] = m;
+  }
   // the previous step's close, by the workgroup of the case's first tile once
   // its own tile is done.  The other tiles of this launch read `active` only
   // (a tile that sees the old value computes a residual that the next launch,

@@ -66,8 +66,8 @@ typedef enum fvm_precision {
 } fvm_precision;
 
 typedef enum fvm_kernels {
-    FVM_KERNELS_TILED = 0, /* one kernel per RK stage over LDS-staged cell tiles (default); with
-                              the global time step, two kernels per stage */
+    FVM_KERNELS_TILED = 0, /* one kernel per RK stage over LDS-staged cell tiles (default), with
+                              the local or the global time step */
     FVM_KERNELS_PLAIN = 1, /* the reference's kernel structure: one kernel per phase, edge
                               slots in global memory (FP64 only) */
     FVM_KERNELS_SPLIT = 2  /* two kernels per RK stage (flux, then smoothing + update) over the

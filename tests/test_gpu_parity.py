@@ -91,11 +91,16 @@ def test_fp32_one_and_two_kernel_paths_agree(naca3k):
     b = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp32", kernels="split")
     c = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp32", kernels="tiled", tile_cells=48,
                threads=128)
+    # a thread with three or more staged entries, sweep entries and edges
+    d = Solver(naca3k, 0.8, 1.25, cfl=1.5, precision="fp32", kernels="tiled", tile_cells=160,
+               threads=64)
     ha, _, _ = a.run(50)
     hb, _, _ = b.run(50)
     c.run(50)
+    d.run(50)
     assert np.array_equal(a.state(), b.state())
     assert np.array_equal(a.state(), c.state())
+    assert np.array_equal(a.state(), d.state())
     np.testing.assert_allclose(ha, hb, rtol=1e-13, atol=0)
 
 
