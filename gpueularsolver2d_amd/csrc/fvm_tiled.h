@@ -764,6 +764,18 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     // 1. the previous stage's residual of the tile and rings 1..4 (1..2
     //    without smoothing)
     const int n1 = a.smoothing ? N4 : NW;
+    // Global time step: a thread's first two of the case's tile minima of the
+    // previous stage are requested here, ahead of the residual staging, and
+    // combined after it (further ones, for cases of more than 2 blockDim
+    // tiles, after the staging barrier).  FP32 also requests the 1 / A of its
+    // two staged entries with their residuals, for the update.
+    R gm0 = R(9999999.0), gm1 = R(9999999.0);  // the reference's initial minimum
+    R ia0 = R(1), ia1 = R(1);
+    if (a.global_dt) {
+      const int n = cc.ntile, t0 = cc.tile0, k2 = s0 + (int)blockDim.x;
+      if (s0 < n) gm0 = a.tile_min_in[t0 + s0];
+      if (k2 < n) gm1 = a.tile_min_in[t0 + k2];
+    }
     // The smoothing sums `if (neighbour is a cell) q += R[neighbour]` read a
     // boundary side from a slot past the end of R (and of R1) that holds -0:
     // adding -0 leaves every q unchanged (+0 + -0 = +0, -0 + -0 = -0, and q
@@ -788,6 +800,12 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
       const int g2 = j2 < nt ? d.cell0 + j2 : rid2;
       const SVec4<R> r = Rin[g];
       const SVec4<R> r2 = Rin[g2];
+      if constexpr (sizeof(R) == 4) {
+        if (a.global_dt) {
+          ia0 = area_pair(cg, g).a;
+          ia1 = area_pair(cg, g2).a;
+        }
+      }
       // both loads are issued here, before the stores that use them (left to
       // itself the compiler moves each load into its store's branch, where
       // the second waits for the first)
@@ -810,12 +828,10 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     if (!a.early_exit && threadIdx.x == 0) s_active = cc.active;
     __syncthreads();
     if (!s_active) return;
-    R gm0 = R(9999999.0), gm1 = R(9999999.0);  // the reference's initial minimum
     if (a.global_dt) {
-      const int n = cc.ntile, t0 = cc.tile0, k2 = s0 + (int)blockDim.x;
-      if (s0 < n) gm0 = a.tile_min_in[t0 + s0];
-      if (k2 < n) gm1 = a.tile_min_in[t0 + k2];
-      for (int k = k2 + (int)blockDim.x; k < n; k += blockDim.x) gm1 = fmin(gm1, a.tile_min_in[t0 + k]);
+      const int n = cc.ntile, t0 = cc.tile0;
+      for (int k = s0 + 2 * (int)blockDim.x; k < n; k += blockDim.x)
+        gm1 = fmin(gm1, a.tile_min_in[t0 + k]);
     }
     // 2. smoothing sweep 1 on the tile and rings 1..3
     if (a.smoothing) {
@@ -870,7 +886,8 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
       R dtv;
       if (a.global_dt) {
         dtv = dtg;
-        if constexpr (sizeof(R) == 4) dtv *= (j == s0 && j < nt) ? ar0.a : area_pair(cg, g).a;
+        if constexpr (sizeof(R) == 4)
+          dtv *= j == s0 ? ia0 : (j == s0 + (int)blockDim.x ? ia1 : area_pair(cg, g).a);
       } else {
         dtv = a.dt_in[d.st0 + g];
       }

@@ -35,6 +35,24 @@ def test_batch_equals_single_solves(meshes, precision, kernels):
         assert (cl[i], cd[i]) == s.forces()
 
 
+@pytest.mark.parametrize("kernels", ["tiled", "split"])
+def test_global_dt_batch_matches_oracle_per_case(meshes, kernels):
+    # each case advances with its own minimum over its own tiles; small tiles
+    # give every case several of them, and the oracle checks each case alone
+    case_mesh = [1, 0, 1]
+    flows = [FLOWS[1], FLOWS[2], FLOWS[0]]
+    b = Batch(meshes, case_mesh, flows, precision="fp64", kernels=kernels, global_dt=True,
+              tile_cells=64, threads=128)
+    hist, steps, status = b.run(12)
+    assert list(steps) == [12] * 3 and not status.any()
+    for i, (m, f) in enumerate(zip(case_mesh, flows)):
+        om = OracleMesh(meshes[m].x, meshes[m].y, meshes[m].tri)
+        o = OracleSolver(om, f["mach"], f["alpha_deg"], cfl=f["cfl"], global_dt=True)
+        ho, _ = o.run(12)
+        assert np.array_equal(b.state(i), o.state()), (kernels, i)
+        np.testing.assert_allclose(hist[i, 1:13], ho[1:], rtol=1e-12, atol=0)
+
+
 def test_batch_fp64_matches_oracle(meshes):
     b = Batch(meshes, [1, 0], [FLOWS[1], FLOWS[2]], precision="fp64")
     b.run(15)
