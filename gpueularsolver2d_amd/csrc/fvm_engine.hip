@@ -297,6 +297,7 @@ class Batch {
   bool ctl_on_host_ = false;         // the host's control blocks are newer than the device's
   hipEvent_t ev_ctl_[2] = {nullptr, nullptr};
   size_t lds_a_ = 0, lds_b_ = 0, lds_s_ = 0;
+  int slot_off_ = 0;  // stage kernel: byte offset of the gather slot table in LDS
   bool fused_ = false;      // one kernel per RK stage
   bool r0_valid_ = false;   // (fused) the residual of stage 0 matches the current state
   int max_r4_local_ = 0, max_ring_local_ = 0, max_w_local_ = 0;
@@ -320,7 +321,6 @@ class Batch {
   FlowT<double>* d_flow64_ = nullptr;
   void *d_cg_ = nullptr, *d_edges_ = nullptr;
   int4 *d_nbr_ = nullptr, *d_nloc_ = nullptr;
-  int2* d_slot_ = nullptr;
   int* d_halo_ = nullptr;
   TileDesc* d_tiles_ = nullptr;
   void *d_w0_ = nullptr, *d_wc_ = nullptr, *d_res_ = nullptr, *d_dt_ = nullptr;
@@ -469,17 +469,18 @@ Batch::~Batch() {
 // LDS per workgroup of the kernels this batch launches, for its tile tables.
 template <typename R>
 size_t Batch::tile_lds() const {
-  int ML = 0, ME = 0, M4 = 0, MR = 0, MW = 0;
+  int ML = 0, ME = 0, M4 = 0, MR = 0, MW = 0, MT = 0;
   for (auto& t : tiled_) {
     ML = std::max(ML, t.max_local);
     ME = std::max(ME, t.max_edges);
     M4 = std::max(M4, t.max_r4_local);
     MR = std::max(MR, t.max_ring_local);
     MW = std::max(MW, t.max_w_local);
+    MT = std::max(MT, t.max_tile);
   }
   const int SL = stage_slope_stride(ML);
-  return fused_ ? stage_tile_lds<R>(M4, MR, MW, SL > 0 ? SL : ML, ME)
-                : std::max(flux_tile_lds<R>(ML, ME), update_tile_lds<R>(ML));
+  return fused_ ? stage_tile_lds<R>(M4, MR, MW, SL > 0 ? SL : ML, ML, ME, MT).bytes
+                : std::max(flux_tile_lds<R>(ML, ME, MT), update_tile_lds<R>(ML));
 }
 
 // The largest tile + ring-1 count over the batch's meshes.
@@ -544,23 +545,28 @@ int Batch::auto_tile_cells() {
 template <typename R>
 void Batch::setup_tiled() {
   const int ncase = (int)cases_.size(), nmesh = (int)topo_.size();
-  int ML = 0, ME = 0;
+  int ML = 0, ME = 0, MT = 0;
   for (auto& t : tiled_) {
     ML = std::max(ML, t.max_local);
     ME = std::max(ME, t.max_edges);
+    MT = std::max(MT, t.max_tile);
   }
   for (auto& t : tiled_) {
     max_r4_local_ = std::max(max_r4_local_, t.max_r4_local);
     max_ring_local_ = std::max(max_ring_local_, t.max_ring_local);
     max_w_local_ = std::max(max_w_local_, t.max_w_local);
   }
-  // gather slots (2 * edge + side) and tile-local ids travel as 16-bit fields
-  if (2 * ME + 1 > 32767 || max_r4_local_ > 32767)
+  // gather slots (2 * edge + side) and tile-local ids travel as 16-bit fields;
+  // the FP32 edge record holds the left and right ids of tile + ring 1 in 14 bits
+  if (2 * ME + 1 > 32767 || max_r4_local_ > 32767 || (sizeof(R) == 4 && ML > 8191))
     throw Error{FVM_ERR_ARGUMENT, "tile_cells too large for 16-bit tile-local indices"};
-  lds_a_ = flux_tile_lds<R>(ML, ME);
+  lds_a_ = flux_tile_lds<R>(ML, ME, MT);
   lds_b_ = update_tile_lds<R>(ML);
   const int SL = stage_slope_stride(ML);
-  lds_s_ = stage_tile_lds<R>(max_r4_local_, max_ring_local_, max_w_local_, SL > 0 ? SL : ML, ME);
+  const StageLds sl = stage_tile_lds<R>(max_r4_local_, max_ring_local_, max_w_local_, SL > 0 ? SL : ML, ML,
+                                        ME, MT);
+  lds_s_ = sl.bytes;
+  slot_off_ = (int)sl.slot_off;
   const size_t lds_max = std::max(lds_a_, fused_ ? lds_s_ : lds_b_);
   if (lds_max > 160 * 1024)
     throw Error{FVM_ERR_ARGUMENT, "tile_cells too large: the tile kernels need " +
@@ -634,7 +640,6 @@ void Batch::setup_tiled() {
     cg = arena_.take<CellGeom<R>>(ncell_all);
     d_cg_ = cg;
     d_nbr_ = arena_.take<int4>(ncell_all);
-    d_slot_ = arena_.take<int2>(ncell_all);
     d_nloc_ = arena_.take<int4>(ncell_all);
     d_halo_ = arena_.take<int>(nhalo_all);
     edges = arena_.take<EdgeRec<R>>(nedge_all);
@@ -680,7 +685,6 @@ void Batch::setup_tiled() {
   }
   std::vector<CellGeom<R>> hcg(ncell_all);
   std::vector<int4> hnbr(ncell_all), hnloc(ncell_all);
-  std::vector<int2> hslot(ncell_all);
   std::vector<int> hhalo(nhalo_all);
   std::vector<EdgeRec<R>> hedge(nedge_all);
   std::vector<int> hwcell(nwall_all);
@@ -705,14 +709,13 @@ void Batch::setup_tiled() {
       g.inv_area = (R)T.inv_area[i];
       g.area = (R)T.area[i];
       hnbr[geo0[m] + i] = make_int4(T.nbr[3 * i], T.nbr[3 * i + 1], T.nbr[3 * i + 2], T.clog[i]);
-      hslot[geo0[m] + i] = pack_tri16(T.slot[3 * i], T.slot[3 * i + 1], T.slot[3 * i + 2]);
       hnloc[geo0[m] + i] = make_int4(T.nloc[3 * i], T.nloc[3 * i + 1], T.nloc[3 * i + 2], 0);
     }
     std::copy(T.halo.begin(), T.halo.end(), hhalo.begin() + halo0[m]);
     for (size_t s = 0; s < T.e_l.size(); s++) {
       hedge[edge0[m] + s] = make_edge_rec(T.e_nx[s], T.e_ny[s], T.e_vn[s], T.e_vx[s], T.e_vy[s],
                                           T.e_rlx[s], T.e_rly[s], T.e_rrx[s], T.e_rry[s], T.e_l[s],
-                                          T.e_r[s], (R*)nullptr);
+                                          T.e_r[s], T.e_kl[s], T.e_kr[s], (R*)nullptr);
     }
     for (size_t w = 0; w < T.w_cell.size(); w++) {
       hwcell[wall0[m] + w] = T.w_cell[w];
@@ -740,7 +743,6 @@ void Batch::setup_tiled() {
   h2d(d_flow64_, hflow64, stream_);
   h2d(cg, hcg, stream_);
   h2d(d_nbr_, hnbr, stream_);
-  h2d(d_slot_, hslot, stream_);
   h2d(d_nloc_, hnloc, stream_);
   h2d(d_halo_, hhalo, stream_);
   h2d(edges, hedge, stream_);
@@ -921,7 +923,6 @@ TiledArgs<R> Batch::tiled_args() {
   a.flow = static_cast<const FlowT<R>*>(d_flow_);
   a.cg = static_cast<const CellGeom<R>*>(d_cg_);
   a.nbr = d_nbr_;
-  a.slot = d_slot_;
   a.nloc = d_nloc_;
   a.halo = d_halo_;
   a.edges = static_cast<const EdgeRec<R>*>(d_edges_);
@@ -949,6 +950,7 @@ TiledArgs<R> Batch::tiled_args() {
   a.max_ring_local = max_ring_local_;
   a.max_w_local = max_w_local_;
   a.early_exit = cases_.size() > 1 ? 1 : 0;
+  a.slot_off = slot_off_;
   a.hist = d_hdesc_;
   return a;
 }

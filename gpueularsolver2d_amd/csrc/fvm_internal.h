@@ -66,8 +66,10 @@ struct TiledMesh {
   std::vector<int> tile_edge0;    // [ntile+1] into the tile edge records
   // tile edge records
   std::vector<int> e_l, e_r;      // tile-local left / right (right -1 far, -2 wall)
+  std::vector<int> e_kl, e_kr;    // the edge's local index (0..2) in its left / right cell
   std::vector<double> e_nx, e_ny, e_vn, e_vx, e_vy, e_rlx, e_rly, e_rrx, e_rry;
   int max_local = 0, max_edges = 0;  // largest tile-local cell and edge counts
+  int max_tile = 0;                  // largest tile's own cell count
   // Rings for the one-kernel-per-stage path: ring 1 is the halo above, ring 2
   // the interior neighbours of ring 1 outside the tile and ring 1, rings 3 and
   // 4 likewise.  The tile-local index space is

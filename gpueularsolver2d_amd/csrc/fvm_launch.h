@@ -31,10 +31,13 @@ enum TileKernel { kFluxTile = 0, kUpdateTile = 1, kStageTile = 2 };
 template <typename R>
 hipError_t tile_blocks_per_cu(TileKernel k, int threads, size_t lds, int max_local, int* blocks);
 
+// max_tile: the largest tile's cell count (the gather slot table, four
+// 16-bit entries per tile cell, follows the edge arrays, 8-byte aligned).
 template <typename R>
-inline size_t flux_tile_lds(int max_local, int max_edges) {
-  return (size_t)max_local * (sizeof(SVec4<R>) + 16 * sizeof(R)) +
-         (size_t)max_edges * (sizeof(SVec4<R>) + sizeof(R));
+inline size_t flux_tile_lds(int max_local, int max_edges, int max_tile) {
+  const size_t sr_end = (size_t)max_local * (sizeof(SVec4<R>) + 16 * sizeof(R)) +
+                        (size_t)max_edges * (sizeof(SVec4<R>) + sizeof(R));
+  return ((sr_end + 7) & ~size_t(7)) + (size_t)max_tile * 4 * 2;
 }
 template <typename R>
 inline size_t update_tile_lds(int max_local) {
@@ -50,15 +53,33 @@ inline int stage_slope_stride(int max_local) {
 }
 
 // max_local: the slope row stride (stage_slope_stride's value, or max_local
-// when that is 0).
+// when that is 0); max_tile: the largest tile's cell count.
+//
+// The gather slot table (8 bytes per tile cell) goes where it costs no LDS
+// if it can: ending at the end of the cell-id array, over W's ring-2 entries
+// and the cell ids, which the flux and gather phases do not read; it must
+// then start at or after W's tile + ring 1 entries (max_local of them, the
+// true count here).  Otherwise it follows SR in the union.
+struct StageLds {
+  size_t bytes;     // dynamic LDS per workgroup
+  size_t slot_off;  // byte offset of the gather slot table
+};
 template <typename R>
-inline size_t stage_tile_lds(int max_r4_local, int max_ring_local, int max_w_local, int max_local,
-                             int max_edges) {
+inline StageLds stage_tile_lds(int max_r4_local, int max_ring_local, int max_w_local, int max_local,
+                               int true_max_local, int max_edges, int max_tile) {
   // W[MW] | gid[MW rounded up to 4] | union { R[M4 + 1] | R1[MR + 1] ; grad + bounds + F + SR }
+  const size_t head = (size_t)max_w_local * sizeof(SVec4<R>) + (size_t)((max_w_local + 3) & ~3) * sizeof(int);
   const size_t res = (size_t)(max_r4_local + max_ring_local + 2) * sizeof(SVec4<R>);
-  const size_t flux = flux_tile_lds<R>(max_local, max_edges) - (size_t)max_local * sizeof(SVec4<R>);
-  return (size_t)max_w_local * sizeof(SVec4<R>) + (size_t)((max_w_local + 3) & ~3) * sizeof(int) +
-         (res > flux ? res : flux);
+  const size_t table = (size_t)max_tile * 4 * 2;
+  const size_t flux_end = (size_t)max_local * 16 * sizeof(R) + (size_t)max_edges * (sizeof(SVec4<R>) + sizeof(R));
+  if (head >= table && head - table >= (size_t)true_max_local * sizeof(SVec4<R>)) {
+    const size_t off = (head - table) & ~size_t(7);
+    if (off >= (size_t)true_max_local * sizeof(SVec4<R>))
+      return StageLds{head + (res > flux_end ? res : flux_end), off};
+  }
+  const size_t off = head + ((flux_end + 7) & ~size_t(7));
+  const size_t union_bytes = ((flux_end + 7) & ~size_t(7)) + table;
+  return StageLds{head + (res > union_bytes ? res : union_bytes), off};
 }
 
 }  // namespace fvm

@@ -255,7 +255,7 @@ TiledMesh build_tiled(const Topology& t, int tile_cells) {
     h.erase(std::unique(h.begin(), h.end()), h.end());
     for (size_t q = 0; q < h.size(); q++) local_of[h[q]] = nt + (int)q;
     T.halo.insert(T.halo.end(), h.begin(), h.end());
-    std::vector<int> tile_edges;
+    std::vector<int> tile_edges, ekl, ekr;
     for (int i = c0; i < c1; i++) {
       const int o = ids[i];
       for (int j = 0; j < 3; j++) {
@@ -263,16 +263,23 @@ TiledMesh build_tiled(const Topology& t, int tile_cells) {
         if (edge_local[e] < 0) {
           edge_local[e] = (int)tile_edges.size();
           tile_edges.push_back(e);
+          ekl.push_back(0);
+          ekr.push_back(0);
         }
         const int side = (t.left[e] == o) ? 0 : 1;
         T.slot[3 * i + j] = 2 * edge_local[e] + side;
+        // the edge's local index in this tile cell, on its side
+        (side == 0 ? ekl : ekr)[edge_local[e]] = j;
         const int nb = T.nbr[3 * i + j];
         T.nloc[3 * i + j] = nb >= 0 ? local_of[nb] : -1;
       }
     }
-    for (int e : tile_edges) {
+    for (size_t q = 0; q < tile_edges.size(); q++) {
+      const int e = tile_edges[q];
       const int L = T.old_to_new[t.left[e]];
       T.e_l.push_back(local_of[L]);
+      T.e_kl.push_back(ekl[q]);
+      T.e_kr.push_back(ekr[q]);
       if (t.right[e] >= 0) {
         T.e_r.push_back(local_of[T.old_to_new[t.right[e]]]);
       } else {
@@ -298,6 +305,7 @@ TiledMesh build_tiled(const Topology& t, int tile_cells) {
       }
     }
     T.max_local = std::max(T.max_local, nt + (int)h.size());
+    T.max_tile = std::max(T.max_tile, nt);
     T.max_edges = std::max(T.max_edges, (int)tile_edges.size());
     for (int e : tile_edges) edge_local[e] = -1;
     for (int i = c0; i < c1; i++) local_of[i] = -1;
@@ -396,6 +404,13 @@ TiledMesh build_tiled(const Topology& t, int tile_cells) {
         const int sl = T.slot[3 * i + j], nb = T.nbr[3 * i + j], lo = T.nloc[3 * i + j];
         ok = sl >= 0 && (sl >> 1) < ne && nb >= -2 && nb < C && lo >= -1 && lo < nl &&
              ((nb >= 0) == (lo >= 0));
+        // the tile edge in that slot names this cell and this local index on
+        // that side, so the kernels can rebuild the slot from the edge record
+        if (ok) {
+          const int q = T.tile_edge0[k] + (sl >> 1);
+          ok = (sl & 1) == 0 ? (T.e_l[q] == i - c0 && T.e_kl[q] == j)
+                             : (T.e_r[q] == i - c0 && T.e_kr[q] == j);
+        }
       }
     if (!ok) throw Error{FVM_ERR_MESH, "internal: tile " + std::to_string(k) + " index check failed"};
   }

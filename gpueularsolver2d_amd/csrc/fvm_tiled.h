@@ -92,7 +92,9 @@ template <typename R>
 struct EdgeRec;
 
 // FP64: the edge vector itself is kept because the spectral radius of the
-// reference uses it directly (u vy + v vx, vx^2 + vy^2).
+// reference uses it directly (u vy + v vx, vx^2 + vy^2).  l and r carry the
+// local ids in their low 16 bits (r signed) and the edge's local index in
+// the left / right cell (0..2) above them.
 template <>
 struct alignas(16) EdgeRec<double> {
   double nx, ny, vn, vx, vy, rlx, rly, rrx, rry;
@@ -100,24 +102,31 @@ struct alignas(16) EdgeRec<double> {
 };
 
 // FP32: 32 bytes; the edge vector is recovered as (ny vn, nx vn) and the two
-// local indices are packed as 16-bit halves.
+// local indices and the edge's local index in each of its cells are packed
+// in one word.
 template <>
 struct alignas(16) EdgeRec<float> {
   float nx, ny, vn, rlx, rly, rrx, rry;
-  int lr;  // low 16 bits: left (unsigned), high 16 bits: right (signed)
+  int lr;  // bits 0-13: left (unsigned), 14-15: its local edge index; bits 16-29: right
+           // (signed: -1 far field, -2 wall), 30-31: its local edge index
 };
 
 __host__ __device__ inline EdgeRec<double> make_edge_rec(double nx, double ny, double vn, double vx,
                                                          double vy, double rlx, double rly,
                                                          double rrx, double rry, int l, int r,
-                                                         double*) {
-  return EdgeRec<double>{nx, ny, vn, vx, vy, rlx, rly, rrx, rry, l, r};
+                                                         int kl, int kr, double*) {
+  return EdgeRec<double>{nx, ny, vn, vx, vy, rlx, rly, rrx, rry,
+                         (int)(((unsigned)kl << 16) | ((unsigned)l & 0xffffu)),
+                         (int)(((unsigned)kr << 16) | ((unsigned)r & 0xffffu))};
 }
 __host__ __device__ inline EdgeRec<float> make_edge_rec(double nx, double ny, double vn, double, double,
                                                         double rlx, double rly, double rrx,
-                                                        double rry, int l, int r, float*) {
+                                                        double rry, int l, int r, int kl, int kr,
+                                                        float*) {
   return EdgeRec<float>{(float)nx, (float)ny, (float)vn, (float)rlx, (float)rly, (float)rrx,
-                        (float)rry, (int)(((unsigned)r << 16) | ((unsigned)l & 0xffffu))};
+                        (float)rry,
+                        (int)(((unsigned)kr << 30) | (((unsigned)r & 0x3fffu) << 16) |
+                              ((unsigned)kl << 14) | ((unsigned)l & 0x3fffu))};
 }
 
 // Spectral radius of a tile edge: FP64 keeps the reference's form on the
@@ -132,11 +141,13 @@ __device__ __forceinline__ float edge_sr(const EdgeRec<float>& e, const Vec4<flo
 }
 
 struct EdgeView {
-  int l, r;
+  int l, r, kl, kr;
 };
-__device__ __forceinline__ EdgeView edge_cells(const EdgeRec<double>& e) { return {e.l, e.r}; }
+__device__ __forceinline__ EdgeView edge_cells(const EdgeRec<double>& e) {
+  return {e.l & 0xffff, (e.r << 16) >> 16, e.l >> 16, (e.r >> 16) & 3};
+}
 __device__ __forceinline__ EdgeView edge_cells(const EdgeRec<float>& e) {
-  return {e.lr & 0xffff, e.lr >> 16};
+  return {e.lr & 0x3fff, (e.lr << 2) >> 18, (e.lr >> 14) & 3, (int)((unsigned)e.lr >> 30)};
 }
 __device__ __forceinline__ double edge_vx(const EdgeRec<double>& e) { return e.vx; }
 __device__ __forceinline__ double edge_vy(const EdgeRec<double>& e) { return e.vy; }
@@ -151,7 +162,6 @@ struct TiledArgs {
   const FlowT<R>* flow;       // per case
   const CellGeom<R>* cg;      // per mesh cell (all meshes concatenated)
   const int4* nbr;            // per mesh cell: neighbours (new ids, -1 far, -2 wall), .w = CLog
-  const int2* slot;           // per mesh cell: 2*slot + side of the three local edges (Tri16)
   const int4* nloc;           // per mesh cell: tile-local index of interior neighbours or -1
   const int* halo;
   const EdgeRec<R>* edges;
@@ -183,6 +193,7 @@ struct TiledArgs {
   int parity;                 // step parity within the run: step-start buffer = w0_cur ^ parity
   int close_prev;             // 1: the workgroup of each case's first tile closes the previous step
   int early_exit;             // 1: test a case's active flag before any load (several cases)
+  int slot_off;               // stage_tile: byte offset of the gather slot table in LDS
   const HistDesc* hist;       // residual history
 };
 
@@ -385,12 +396,16 @@ __device__ __forceinline__ void gradient_cell(int j, const Vec4<R>& Wi, const Ce
 // Roe flux and spectral radius of every edge of the tile into LDS slots
 // (kernel_Flux2 gpu_solver.cu:651-764 with kernel_LBJinterpolate :496-549,
 // kernel_deltT :385-454).  kPre: e0 is this thread's prefetched first record
-// (otherwise unused).
+// (otherwise unused).  Each edge also enters its slot, 2 * edge + side, in the
+// gather table sSl[4 * cell + local edge index] of each of its tile cells
+// (cells of the tile have local ids below d.ncell), so the gather reads no
+// slot table from memory (four entries per cell, the fourth unused, so that
+// the gather reads a cell's three with one 8-byte access).
 template <typename R, bool kPre = true>
 __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc& d,
                                            const FlowT<R>& f, const SVec4<R>* sW, const R* sG,
-                                           const R* sB, SVec4<R>* sF, R* sS, int ML,
-                                           const EdgeRec<R>& e0) {
+                                           const R* sB, SVec4<R>* sF, R* sS,
+                                           unsigned short* sSl, int ML, const EdgeRec<R>& e0) {
   const int s0 = threadIdx.x;
   // FP32: the record of this thread's next edge is requested before this
   // edge is computed, so an edge beyond the first blockDim costs no round
@@ -407,6 +422,8 @@ __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc
       e = (kPre && s == s0) ? e0 : a.edges[d.edge0 + s];
     }
     const EdgeView ev = edge_cells(e);
+    if (ev.l < d.ncell) sSl[4 * ev.l + ev.kl] = (unsigned short)(2 * s);
+    if (ev.r >= 0 && ev.r < d.ncell) sSl[4 * ev.r + ev.kr] = (unsigned short)(2 * s + 1);
     const R vx = edge_vx(e), vy = edge_vy(e);
     const Vec4<R> WL = ld(sW[ev.l]);
     Vec4<R> L = WL, Rr, Rraw;
@@ -458,17 +475,18 @@ __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc
 // step).  The local time step is stored as dt in FP64 and as dt / A in FP32
 // (the RK update's only use of dt is dt R / A, so FP32 folds the division by
 // the area in here and the update reads no area).  ar0: this thread's first
-// cell's (1 / A, A), read at kernel start.
+// cell's (1 / A, A), read at kernel start.  sSl: the slots the flux phase
+// entered (2 * edge + side of each cell's three local edges).
 template <typename R>
 __device__ __forceinline__ R gather_cells(const TiledArgs<R>& a, const TileDesc& d, const FlowT<R>& f,
-                                          const SVec4<R>* sF, const R* sS, SVec4<R>* Rout, R* dtout,
-                                          const int2& sl0, Pair<R> ar0) {
+                                          const SVec4<R>* sF, const R* sS, const unsigned short* sSl,
+                                          SVec4<R>* Rout, R* dtout, Pair<R> ar0) {
   const int s0 = threadIdx.x;
   R tmin = R(3.0e38);
   for (int i = threadIdx.x; i < d.ncell; i += blockDim.x) {
     const int g = d.cell0 + i;
-    const int4 sl = unpack_tri16((i == s0) ? sl0 : a.slot[d.geo0 + g]);
-    const int slk[3] = {sl.x, sl.y, sl.z};
+    const uint2 sp = reinterpret_cast<const uint2*>(sSl)[i];
+    const int slk[3] = {(int)(sp.x & 0xffffu), (int)(sp.x >> 16), (int)(sp.y & 0xffffu)};
     R T = 0;
     R r0 = R(0.0), r1 = R(0.0), r2 = R(0.0), r3 = R(0.0);
 #pragma unroll
@@ -521,7 +539,7 @@ __device__ __forceinline__ double block_sum(double acc) {
 
 // --------------------------------------------------------------------------
 // Kernel A: gradient + flux + gather for one tile.
-// LDS: W[nl] | grad[8][nl] | bounds[8][nl] | F[ne] | SR[ne]
+// LDS: W[nl] | grad[8][nl] | bounds[8][nl] | F[ne] | SR[ne] | gather slots[4][nt] (16-bit)
 // --------------------------------------------------------------------------
 template <typename R>
 __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
@@ -537,6 +555,9 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
   R* sB = sG + 8 * ML;                    // [4][ML] (max, min) pairs
   SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
   R* sS = reinterpret_cast<R*>(sF + ME);
+  // [4][ncell] gather slots, 8-byte aligned (read as one 8-byte access per cell)
+  unsigned short* sSl = reinterpret_cast<unsigned short*>(
+      smem + ((reinterpret_cast<unsigned char*>(sS + ME) - smem + 7) & ~7));
   const SVec4<R>* W = a.w_in + d.st0;
 
   // prefetch this thread's first edge record and gather slot; their latency
@@ -544,12 +565,8 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
   const int s0 = threadIdx.x;
   EdgeRec<R> e0{};
   if (s0 < d.nedge) e0 = a.edges[d.edge0 + s0];
-  int2 sl0 = make_int2(0, 0);
   Pair<R> ar0{R(0), R(0)};
-  if (s0 < nt) {
-    sl0 = a.slot[d.geo0 + d.cell0 + s0];
-    ar0 = area_pair(a.cg + d.geo0, d.cell0 + s0);
-  }
+  if (s0 < nt) ar0 = area_pair(a.cg + d.geo0, d.cell0 + s0);
 
   for (int j = threadIdx.x; j < nl; j += blockDim.x) {
     const int g = j < nt ? d.cell0 + j : a.halo[d.halo0 + j - nt];
@@ -565,9 +582,9 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
     }
   }
   __syncthreads();
-  flux_edges<R>(a, d, f, sW, sG, sB, sF, sS, ML, e0);
+  flux_edges<R>(a, d, f, sW, sG, sB, sF, sS, sSl, ML, e0);
   __syncthreads();
-  const R tmin = gather_cells<R>(a, d, f, sF, sS, a.res, a.dt, sl0, ar0);
+  const R tmin = gather_cells<R>(a, d, f, sF, sS, sSl, a.res, a.dt, ar0);
   if (a.global_dt) {
     const R m = block_min(tmin);
     if (threadIdx.x == 0) a.tile_min[t] = m;
@@ -681,6 +698,8 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
 // --------------------------------------------------------------------------
 // One kernel per RK stage (see the header comment).
 // LDS: W[MW] | gid[MW4] | union { R[M4 + 1] | R1[MR + 1] ;  grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME] }
+// and the gather slots[4][tile cells] (16-bit) at slot_off: ending where gid ends when that
+// leaves W's entries of tile + ring 1 clear, else after SR
 // (ML: the slope row stride, S or max_local; MW4: MW rounded up to 4)
 // R holds the previous stage's residual of the tile and rings 1..4, one load
 // per cell, so that smoothing sweep 1 gathers from LDS through the 16-bit
@@ -730,6 +749,11 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   R* sB = sG + 8 * ML;
   SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
   R* sS = reinterpret_cast<R*>(sF + ME);
+  // [4][ncell] gather slots, 8-byte aligned (one 8-byte read per cell): where
+  // the host found room, over the tail of W (rings 2, which only the gradient
+  // reads) and the cell ids (read by sweep 2 and the gradient), both dead in
+  // the flux and gather phases that use the table; else after SR
+  unsigned short* sSl = reinterpret_cast<unsigned short*>(smem + a.slot_off);
   const int cur = cc.w0_cur ^ a.parity;
   const SVec4<R>* W0 = (cur ? a.w0b : a.w0a) + d.st0;
   SVec4<R>* W0n = (cur ? a.w0a : a.w0b) + d.st0;
@@ -744,12 +768,8 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   const int s0 = threadIdx.x;
   EdgeRec<R> e0{};
   if (kPreEdge && s0 < d.nedge) e0 = a.edges[d.edge0 + s0];
-  int2 sl0 = make_int2(0, 0);
   Pair<R> ar0{R(0), R(0)};
-  if (s0 < nt) {
-    sl0 = a.slot[d.geo0 + d.cell0 + s0];
-    ar0 = area_pair(cg, d.cell0 + s0);
-  }
+  if (s0 < nt) ar0 = area_pair(cg, d.cell0 + s0);
 
   // this thread's first local-neighbour entry, read here with the other
   // prefetches and used by sweeps 1 and 2 and the gradient (a thread takes
@@ -974,11 +994,11 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   }
   __syncthreads();
   // 5. flux and spectral radius of the tile's edges
-  flux_edges<R, kPreEdge>(a, d, f, sW, sG, sB, sF, sS, ML, e0);
+  flux_edges<R, kPreEdge>(a, d, f, sW, sG, sB, sF, sS, sSl, ML, e0);
   __syncthreads();
   // 6. ordered gather: this stage's residual and time step (global time
   //    step: the tile's minimum of A*CFL/T)
-  const R tmin = gather_cells<R>(a, d, f, sF, sS, a.res, a.dt, sl0, ar0);
+  const R tmin = gather_cells<R>(a, d, f, sF, sS, sSl, a.res, a.dt, ar0);
   if (a.global_dt) {
     const R m = block_min(tmin);
     if (threadIdx.x == 0) a.tile_min[t] = m;
