@@ -707,7 +707,7 @@ void Batch::setup_tiled() {
       const double gs = sizeof(R) == 4 ? 0.5 * T.inv_area[i] : 1.0;
       for (int k = 0; k < 3; k++) g.set(k, T.gvx[3 * i + k] * gs, T.gvy[3 * i + k] * gs);
       g.inv_area = (R)T.inv_area[i];
-      g.area = (R)T.area[i];
+      g.set_area(T.area[i]);
       hnbr[geo0[m] + i] = make_int4(T.nbr[3 * i], T.nbr[3 * i + 1], T.nbr[3 * i + 2], T.clog[i]);
       hnloc[geo0[m] + i] = make_int4(T.nloc[3 * i], T.nloc[3 * i + 1], T.nloc[3 * i + 2], 0);
     }

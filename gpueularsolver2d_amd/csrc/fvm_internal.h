@@ -96,7 +96,7 @@ TiledMesh build_tiled(const Topology& topo, int tile_cells);
 // Per cell: the gradient's three local edge vectors (gx, gy of side k) and
 // the area.  FP64 stores all three; FP32 stores two, scaled by 0.5 / area:
 // the three vectors of a triangle's sides sum to zero (x1 - x2 + x2 - x0 +
-// x0 - x1), so the third is minus the sum of the first two.  24 instead of
+// x0 - x1), so the third is minus the sum of the first two.  20 instead of
 // 32 bytes per cell, read for the tile and its first ring every launch.
 template <typename R>
 struct alignas(16) CellGeom {
@@ -107,10 +107,14 @@ struct alignas(16) CellGeom {
     vx[k] = (R)x;
     vy[k] = (R)y;
   }
+  __host__ __device__ void set_area(double a) { area = (R)a; }
 };
+// FP32 keeps 1 / A only (20 bytes): its time step is formed as dt / A =
+// min(CFL / T, 0.1 / A), and A CFL / T, the global time step's operand, as
+// CFL / (T / A).
 template <>
-struct alignas(8) CellGeom<float> {
-  float vx[2], vy[2], inv_area, area;
+struct alignas(4) CellGeom<float> {
+  float vx[2], vy[2], inv_area;
   __host__ __device__ float gx(int k) const { return k < 2 ? vx[k] : -(vx[0] + vx[1]); }
   __host__ __device__ float gy(int k) const { return k < 2 ? vy[k] : -(vy[0] + vy[1]); }
   __host__ __device__ void set(int k, double x, double y) {
@@ -119,6 +123,7 @@ struct alignas(8) CellGeom<float> {
       vy[k] = (float)y;
     }
   }
+  __host__ __device__ void set_area(double) {}
 };
 
 template <typename R>

@@ -59,10 +59,15 @@ struct alignas(2 * sizeof(R)) Pair {
   R a, b;
 };
 
-// A cell's (1 / area, area), adjacent in its CellGeom record: one load.
+// A cell's (1 / area, area), adjacent in its CellGeom record: one load.  FP32
+// stores 1 / area only (the second value is 0).
 template <typename R>
 __device__ __forceinline__ Pair<R> area_pair(const CellGeom<R>* cg, int i) {
   return *reinterpret_cast<const Pair<R>*>(&cg[i].inv_area);
+}
+template <>
+__device__ __forceinline__ Pair<float> area_pair<float>(const CellGeom<float>* cg, int i) {
+  return Pair<float>{cg[i].inv_area, 0.0f};
 }
 
 // Three signed 16-bit values and a fourth small one in 8 bytes (x: bits
@@ -503,14 +508,17 @@ __device__ __forceinline__ R gather_cells(const TiledArgs<R>& a, const TileDesc&
     const Pair<R> ar = (i == s0) ? ar0 : area_pair(a.cg + d.geo0, g);
     if (a.global_dt) {
       // only the minimum over the case is used (kernel_TimeStep)
-      const R v = ar.b * f.cfl / T;
-      tmin = fmin(tmin, v);
-    } else {
-      const R dt = fmin(ar.b * f.cfl / T, R(0.1));
+      R v;
       if constexpr (sizeof(R) == 4)
-        dtout[d.st0 + g] = dt * ar.a;
+        v = f.cfl / (T * ar.a);  // A CFL / T from the stored 1 / A
       else
-        dtout[d.st0 + g] = dt;
+        v = ar.b * f.cfl / T;
+      tmin = fmin(tmin, v);
+    } else if constexpr (sizeof(R) == 4) {
+      // dt / A = min(A CFL / T, 0.1) / A = min(CFL / T, 0.1 / A)
+      dtout[d.st0 + g] = fmin(f.cfl / T, R(0.1) * ar.a);
+    } else {
+      dtout[d.st0 + g] = fmin(ar.b * f.cfl / T, R(0.1));
     }
   }
   return tmin;
@@ -673,9 +681,11 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
     const R rk = a.rk * dtc;
     const SVec4<R> w0 = a.w0[d.st0 + g];
     SVec4<R> wn;
-    if (sizeof(R) == 4 && !a.global_dt) {
-      // FP32 local time step: dt / A (see gather_cells)
-      wn = SVec4<R>{w0.d - rk * ri.d, w0.du - rk * ri.du, w0.dv - rk * ri.dv, w0.de - rk * ri.de};
+    if constexpr (sizeof(R) == 4) {
+      // FP32: the local time step is stored as dt / A (see gather_cells); the
+      // global one is divided by the area here, through the stored 1 / A
+      const R s = a.global_dt ? rk * a.cg[d.geo0 + g].inv_area : rk;
+      wn = SVec4<R>{w0.d - s * ri.d, w0.du - s * ri.du, w0.dv - s * ri.dv, w0.de - s * ri.de};
     } else {
       const R area = a.cg[d.geo0 + g].area;
       wn.d = w0.d - rk * ri.d / area;
