@@ -106,12 +106,12 @@ struct alignas(16) EdgeRec<double> {
   int l, r;
 };
 
-// FP32: 32 bytes; the edge vector is recovered as (ny vn, nx vn) and the two
-// local indices and the edge's local index in each of its cells are packed
-// in one word.
+// FP32: 28 bytes, the edge vector (its unit normal and length are formed
+// from it: n = (vy, vx) / |v|), the reconstruction offsets, and the two local
+// indices and the edge's local index in each of its cells in one word.
 template <>
-struct alignas(16) EdgeRec<float> {
-  float nx, ny, vn, rlx, rly, rrx, rry;
+struct alignas(4) EdgeRec<float> {
+  float vx, vy, rlx, rly, rrx, rry;
   int lr;  // bits 0-13: left (unsigned), 14-15: its local edge index; bits 16-29: right
            // (signed: -1 far field, -2 wall), 30-31: its local edge index
 };
@@ -124,25 +124,40 @@ __host__ __device__ inline EdgeRec<double> make_edge_rec(double nx, double ny, d
                          (int)(((unsigned)kl << 16) | ((unsigned)l & 0xffffu)),
                          (int)(((unsigned)kr << 16) | ((unsigned)r & 0xffffu))};
 }
-__host__ __device__ inline EdgeRec<float> make_edge_rec(double nx, double ny, double vn, double, double,
+__host__ __device__ inline EdgeRec<float> make_edge_rec(double, double, double, double vx, double vy,
                                                         double rlx, double rly, double rrx,
                                                         double rry, int l, int r, int kl, int kr,
                                                         float*) {
-  return EdgeRec<float>{(float)nx, (float)ny, (float)vn, (float)rlx, (float)rly, (float)rrx,
-                        (float)rry,
+  return EdgeRec<float>{(float)vx, (float)vy, (float)rlx, (float)rly, (float)rrx, (float)rry,
                         (int)(((unsigned)kr << 30) | (((unsigned)r & 0x3fffu) << 16) |
                               ((unsigned)kl << 14) | ((unsigned)l & 0x3fffu))};
 }
 
+// Unit normal n = (vy, vx) / |v| and length |v| of a tile edge: stored in
+// FP64 (computed in FP64 on the host), formed from the stored edge vector in
+// FP32.
+template <typename R>
+struct EdgeNormal {
+  R nx, ny, vn;
+};
+__device__ __forceinline__ EdgeNormal<double> edge_normal(const EdgeRec<double>& e) {
+  return {e.nx, e.ny, e.vn};
+}
+__device__ __forceinline__ EdgeNormal<float> edge_normal(const EdgeRec<float>& e) {
+  const float vn = sqrtf(e.vx * e.vx + e.vy * e.vy);
+  const float iv = 1.0f / vn;
+  return {e.vy * iv, e.vx * iv, vn};
+}
+
 // Spectral radius of a tile edge: FP64 keeps the reference's form on the
 // edge vector, FP32 uses the unit normal and length.
-__device__ __forceinline__ double edge_sr(const EdgeRec<double>&, const Vec4<double>& L,
+__device__ __forceinline__ double edge_sr(const EdgeNormal<double>&, const Vec4<double>& L,
                                           const Vec4<double>& R, double vx, double vy, double g) {
   return spectral_radius(L, R, vx, vy, g);
 }
-__device__ __forceinline__ float edge_sr(const EdgeRec<float>& e, const Vec4<float>& L,
+__device__ __forceinline__ float edge_sr(const EdgeNormal<float>& n, const Vec4<float>& L,
                                          const Vec4<float>& R, float, float, float g) {
-  return spectral_radius_n(L, R, e.nx, e.ny, e.vn, g);
+  return spectral_radius_n(L, R, n.nx, n.ny, n.vn, g);
 }
 
 struct EdgeView {
@@ -156,8 +171,8 @@ __device__ __forceinline__ EdgeView edge_cells(const EdgeRec<float>& e) {
 }
 __device__ __forceinline__ double edge_vx(const EdgeRec<double>& e) { return e.vx; }
 __device__ __forceinline__ double edge_vy(const EdgeRec<double>& e) { return e.vy; }
-__device__ __forceinline__ float edge_vx(const EdgeRec<float>& e) { return e.ny * e.vn; }
-__device__ __forceinline__ float edge_vy(const EdgeRec<float>& e) { return e.nx * e.vn; }
+__device__ __forceinline__ float edge_vx(const EdgeRec<float>& e) { return e.vx; }
+__device__ __forceinline__ float edge_vy(const EdgeRec<float>& e) { return e.vy; }
 
 template <typename R>
 struct TiledArgs {
@@ -430,6 +445,7 @@ __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc
     if (ev.l < d.ncell) sSl[4 * ev.l + ev.kl] = (unsigned short)(2 * s);
     if (ev.r >= 0 && ev.r < d.ncell) sSl[4 * ev.r + ev.kr] = (unsigned short)(2 * s + 1);
     const R vx = edge_vx(e), vy = edge_vy(e);
+    const EdgeNormal<R> en_ = edge_normal(e);
     const Vec4<R> WL = ld(sW[ev.l]);
     Vec4<R> L = WL, Rr, Rraw;
     if (ev.r >= 0) {
@@ -461,16 +477,16 @@ __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc
       Rraw = far_ghost(WL, vx, vy, f);
       Rr = Rraw;
     } else {
-      Rraw = wall_ghost(WL, vx, vy, e.vn);
+      Rraw = wall_ghost(WL, vx, vy, en_.vn);
       Rr = Rraw;
     }
-    Vec4<R> F = roe_flux(L, Rr, e.nx, e.ny, f.gamma);
-    F.d *= e.vn;
-    F.du *= e.vn;
-    F.dv *= e.vn;
-    F.de *= e.vn;
+    Vec4<R> F = roe_flux(L, Rr, en_.nx, en_.ny, f.gamma);
+    F.d *= en_.vn;
+    F.du *= en_.vn;
+    F.dv *= en_.vn;
+    F.de *= en_.vn;
     sF[s] = st(F);
-    sS[s] = edge_sr(e, WL, Rraw, vx, vy, f.gamma);
+    sS[s] = edge_sr(en_, WL, Rraw, vx, vy, f.gamma);
   }
 }
 
