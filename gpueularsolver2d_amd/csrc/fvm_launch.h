@@ -31,10 +31,14 @@ enum TileKernel { kFluxTile = 0, kUpdateTile = 1, kStageTile = 2 };
 template <typename R>
 hipError_t tile_blocks_per_cu(TileKernel k, int threads, size_t lds, int max_local, int* blocks);
 
-// max_tile: the largest tile's cell count (the gather slot table, four
-// 16-bit entries per tile cell, follows the edge arrays, 8-byte aligned).
+// FP64: max_tile is the largest tile's cell count (the gather slot table,
+// four 16-bit entries per tile cell, follows the edge arrays, 8-byte
+// aligned).  FP32: W, then three face states and three spectral radii per
+// cell of the tile and ring 1.
 template <typename R>
 inline size_t flux_tile_lds(int max_local, int max_edges, int max_tile) {
+  if constexpr (sizeof(R) == 4)
+    return (size_t)max_local * (4 * sizeof(SVec4<R>) + 3 * sizeof(R));
   const size_t sr_end = (size_t)max_local * (sizeof(SVec4<R>) + 16 * sizeof(R)) +
                         (size_t)max_edges * (sizeof(SVec4<R>) + sizeof(R));
   return ((sr_end + 7) & ~size_t(7)) + (size_t)max_tile * 4 * 2;
@@ -52,17 +56,19 @@ inline int stage_slope_stride(int max_local) {
   return 0;
 }
 
-// max_local: the slope row stride (stage_slope_stride's value, or max_local
-// when that is 0); max_tile: the largest tile's cell count.
+// max_local: the slope (FP64) or face (FP32) row stride (stage_slope_stride's
+// value, or max_local when that is 0); max_tile: the largest tile's cell count.
 //
-// The gather slot table (8 bytes per tile cell) goes where it costs no LDS
-// if it can: ending at the end of the cell-id array, over W's ring-2 entries
-// and the cell ids, which the flux and gather phases do not read; it must
-// then start at or after W's tile + ring 1 entries (max_local of them, the
-// true count here).  Otherwise it follows SR in the union.
+// FP64: the gather slot table (8 bytes per tile cell) goes where it costs no
+// LDS if it can: ending at the end of the cell-id array, over W's ring-2
+// entries and the cell ids, which the flux and gather phases do not read; it
+// must then start at or after W's tile + ring 1 entries (max_local of them,
+// the true count here).  Otherwise it follows SR in the union.  FP32 has no
+// slot table: its flux phase holds three face slots and three spectral radii
+// per row entry.
 struct StageLds {
   size_t bytes;     // dynamic LDS per workgroup
-  size_t slot_off;  // byte offset of the gather slot table
+  size_t slot_off;  // byte offset of the gather slot table (FP64)
 };
 template <typename R>
 inline StageLds stage_tile_lds(int max_r4_local, int max_ring_local, int max_w_local, int max_local,
@@ -70,6 +76,10 @@ inline StageLds stage_tile_lds(int max_r4_local, int max_ring_local, int max_w_l
   // W[MW] | gid[MW rounded up to 4] | union { R[M4 + 1] | R1[MR + 1] ; grad + bounds + F + SR }
   const size_t head = (size_t)max_w_local * sizeof(SVec4<R>) + (size_t)((max_w_local + 3) & ~3) * sizeof(int);
   const size_t res = (size_t)(max_r4_local + max_ring_local + 2) * sizeof(SVec4<R>);
+  if constexpr (sizeof(R) == 4) {
+    const size_t faces = (size_t)max_local * 3 * (sizeof(SVec4<R>) + sizeof(R));
+    return StageLds{head + (res > faces ? res : faces), 0};
+  }
   const size_t table = (size_t)max_tile * 4 * 2;
   const size_t flux_end = (size_t)max_local * 16 * sizeof(R) + (size_t)max_edges * (sizeof(SVec4<R>) + sizeof(R));
   if (head >= table && head - table >= (size_t)true_max_local * sizeof(SVec4<R>)) {

@@ -274,6 +274,20 @@ TiledMesh build_tiled(const Topology& t, int tile_cells) {
         T.nloc[3 * i + j] = nb >= 0 ? local_of[nb] : -1;
       }
     }
+    // the edge's local index in a ring-1 cell on its other side too (FP32
+    // addresses the face slots of both sides through it), then both checked
+    for (size_t q = 0; q < tile_edges.size(); q++) {
+      const int e = tile_edges[q];
+      for (int side = 0; side < 2; side++) {
+        const int o = side == 0 ? t.left[e] : t.right[e];
+        if (o < 0 || tile_of[T.old_to_new[o]] == k) continue;
+        for (int j = 0; j < 3; j++)
+          if (t.celledge[3 * o + j] == e) (side == 0 ? ekl : ekr)[q] = j;
+      }
+      if (t.celledge[3 * t.left[e] + ekl[q]] != e ||
+          (t.right[e] >= 0 && t.celledge[3 * t.right[e] + ekr[q]] != e))
+        throw Error{FVM_ERR_MESH, "internal: edge local index check failed"};
+    }
     for (size_t q = 0; q < tile_edges.size(); q++) {
       const int e = tile_edges[q];
       const int L = T.old_to_new[t.left[e]];

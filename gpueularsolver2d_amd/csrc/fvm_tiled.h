@@ -91,27 +91,30 @@ __device__ __forceinline__ int side_slot(int lk, int dummy) {
 }
 
 // Per tile edge: unit normal (n = (vy, vx)/|v|, the reference's convention),
-// length, reconstruction offsets mid - centroid of both cells, and the
-// tile-local indices of the two cells (right: -1 far field, -2 wall).
+// length, the tile-local indices of the two cells (right: -1 far field, -2
+// wall) and the edge's local index (0..2) in each of them.
 template <typename R>
 struct EdgeRec;
 
-// FP64: the edge vector itself is kept because the spectral radius of the
-// reference uses it directly (u vy + v vx, vx^2 + vy^2).  l and r carry the
-// local ids in their low 16 bits (r signed) and the edge's local index in
-// the left / right cell (0..2) above them.
+// FP64: also the reconstruction offsets mid - centroid of both cells, and the
+// edge vector itself, because the spectral radius of the reference uses it
+// directly (u vy + v vx, vx^2 + vy^2).  l and r carry the local ids in their
+// low 16 bits (r signed) and the edge's local index in the left / right cell
+// above them.
 template <>
 struct alignas(16) EdgeRec<double> {
   double nx, ny, vn, vx, vy, rlx, rly, rrx, rry;
   int l, r;
 };
 
-// FP32: 28 bytes, the edge vector (its unit normal and length are formed
-// from it: n = (vy, vx) / |v|), the reconstruction offsets, and the two local
-// indices and the edge's local index in each of its cells in one word.
+// FP32: 12 bytes, the edge vector (its unit normal and length are formed
+// from it: n = (vy, vx) / |v|) and, in one word, the two local indices and the
+// edge's local index in each of its cells.  No reconstruction offsets: FP32
+// forms each cell's face states in the gradient phase from the cell's own
+// side vectors (face_states).
 template <>
 struct alignas(4) EdgeRec<float> {
-  float vx, vy, rlx, rly, rrx, rry;
+  float vx, vy;
   int lr;  // bits 0-13: left (unsigned), 14-15: its local edge index; bits 16-29: right
            // (signed: -1 far field, -2 wall), 30-31: its local edge index
 };
@@ -125,10 +128,9 @@ __host__ __device__ inline EdgeRec<double> make_edge_rec(double nx, double ny, d
                          (int)(((unsigned)kr << 16) | ((unsigned)r & 0xffffu))};
 }
 __host__ __device__ inline EdgeRec<float> make_edge_rec(double, double, double, double vx, double vy,
-                                                        double rlx, double rly, double rrx,
-                                                        double rry, int l, int r, int kl, int kr,
-                                                        float*) {
-  return EdgeRec<float>{(float)vx, (float)vy, (float)rlx, (float)rly, (float)rrx, (float)rry,
+                                                        double, double, double, double, int l,
+                                                        int r, int kl, int kr, float*) {
+  return EdgeRec<float>{(float)vx, (float)vy,
                         (int)(((unsigned)kr << 30) | (((unsigned)r & 0x3fffu) << 16) |
                               ((unsigned)kl << 14) | ((unsigned)l & 0x3fffu))};
 }
@@ -338,99 +340,158 @@ __global__ void __launch_bounds__(kFinalizeThreads) finalize_kernel(CaseCtl* ctl
 // Phases shared by flux_tile and stage_tile
 // ---------------------------------------------------------------------------
 
-// Green-Gauss gradient and neighbourhood bounds of one cell into the LDS
-// arrays sG[4][ML] of (d/dx, d/dy) pairs and sB[4][ML] of (max, min) pairs,
-// one row per conservative (kernel_Gradient, gpu_solver.cu:225-383).  The
-// flux reads each pair with one LDS instruction.
+// FP64: Green-Gauss gradient and neighbourhood bounds of one cell into the
+// LDS arrays sG[4][ML] of (d/dx, d/dy) pairs and sB[4][ML] of (max, min)
+// pairs, one row per conservative (kernel_Gradient, gpu_solver.cu:225-383).
+// The flux reads each pair with one LDS instruction.
 // nbk: neighbour codes (>= 0 interior, -1 far field, -2 wall); nbv: the
 // states of the interior neighbours (entries of boundary sides are unused).
 // Callers load all three sides at once, a boundary side reading the cell
 // itself, so the loads overlap instead of waiting on each other.
-template <typename R>
-__device__ __forceinline__ void gradient_cell(int j, const Vec4<R>& Wi, const CellGeom<R>& cg,
-                                              const int nbk[3], const Vec4<R> nbv[3],
-                                              const FlowT<R>& f, R* sG, R* sB, int ML) {
-  R gx[4] = {R(0.0), R(0.0), R(0.0), R(0.0)}, gy[4] = {R(0.0), R(0.0), R(0.0), R(0.0)};
-  R mx[4], mn[4];
-  const R ui[4] = {Wi.d, Wi.du, Wi.dv, Wi.de};
-  // the sides' states, ghosts on the boundary sides.  FP64 puts the ghosts
-  // behind one branch that cells with three interior neighbours (nearly all)
-  // skip together (measured faster there, slower in FP32)
-  Vec4<R> cs[3] = {nbv[0], nbv[1], nbv[2]};
-  if constexpr (sizeof(R) == 8) {
-    if ((nbk[0] | nbk[1] | nbk[2]) < 0) {
+__device__ __forceinline__ void gradient_cell(int j, const Vec4<double>& Wi, const CellGeom<double>& cg,
+                                              const int nbk[3], const Vec4<double> nbv[3],
+                                              const FlowT<double>& f, double* sG, double* sB,
+                                              int ML) {
+  double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
+  double mx[4], mn[4];
+  const double ui[4] = {Wi.d, Wi.du, Wi.dv, Wi.de};
+  // the sides' states, ghosts on the boundary sides, behind one branch that
+  // cells with three interior neighbours (nearly all) skip together
+  Vec4<double> cs[3] = {nbv[0], nbv[1], nbv[2]};
+  if ((nbk[0] | nbk[1] | nbk[2]) < 0) {
 #pragma unroll
-      for (int k = 0; k < 3; k++)
-        if (nbk[k] < 0) cs[k] = boundary_ghost<R>(-nbk[k], Wi, cg.gx(k), cg.gy(k), f);
-    }
+    for (int k = 0; k < 3; k++)
+      if (nbk[k] < 0) cs[k] = boundary_ghost<double>(-nbk[k], Wi, cg.gx(k), cg.gy(k), f);
   }
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    Vec4<R> c;
-    if constexpr (sizeof(R) == 4)
-      c = nbk[k] >= 0 ? nbv[k] : boundary_ghost<R>(-nbk[k], Wi, cg.gx(k), cg.gy(k), f);
-    else
-      c = cs[k];
-    const R cc[4] = {c.d, c.du, c.dv, c.de};
+    const double cc[4] = {cs[k].d, cs[k].du, cs[k].dv, cs[k].de};
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       if (k == 0) {
         mx[q] = cc[q];
         mn[q] = cc[q];
-      } else if constexpr (sizeof(R) == 4) {
-        // FP32: the same extrema by min / max instructions (they may differ
-        // only in the sign of a zero)
-        mn[q] = fminf(mn[q], cc[q]);
-        mx[q] = fmaxf(mx[q], cc[q]);
       } else {
         if (cc[q] < mn[q]) mn[q] = cc[q];
         if (cc[q] > mx[q]) mx[q] = cc[q];
       }
     }
-    if constexpr (sizeof(R) == 4) {
-      // FP32: the edge vectors come scaled by 0.5 / area (the ghosts above
-      // only use their direction)
 #pragma unroll
-      for (int q = 0; q < 4; q++) gx[q] += (ui[q] + cc[q]) * cg.gy(k);
+    for (int q = 0; q < 4; q++) gx[q] += 0.5 * (ui[q] + cc[q]) * cg.gy(k);
 #pragma unroll
-      for (int q = 0; q < 4; q++) gy[q] += (ui[q] + cc[q]) * cg.gx(k);
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; q++) gx[q] += R(0.5) * (ui[q] + cc[q]) * cg.gy(k);
-#pragma unroll
-      for (int q = 0; q < 4; q++) gy[q] += R(0.5) * (ui[q] + cc[q]) * cg.gx(k);
-    }
+    for (int q = 0; q < 4; q++) gy[q] += 0.5 * (ui[q] + cc[q]) * cg.gx(k);
   }
-  Pair<R>* sG2 = reinterpret_cast<Pair<R>*>(sG);
-  Pair<R>* sB2 = reinterpret_cast<Pair<R>*>(sB);
+  Pair<double>* sG2 = reinterpret_cast<Pair<double>*>(sG);
+  Pair<double>* sB2 = reinterpret_cast<Pair<double>*>(sB);
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    if constexpr (sizeof(R) == 4)
-      sG2[q * ML + j] = Pair<R>{gx[q], gy[q]};
-    else
-      sG2[q * ML + j] = Pair<R>{gx[q] * cg.inv_area, gy[q] * cg.inv_area};
-    sB2[q * ML + j] = Pair<R>{fmax(ui[q], mx[q]), fmin(ui[q], mn[q])};
+    sG2[q * ML + j] = Pair<double>{gx[q] * cg.inv_area, gy[q] * cg.inv_area};
+    sB2[q * ML + j] = Pair<double>{fmax(ui[q], mx[q]), fmin(ui[q], mn[q])};
   }
 }
 
-// Roe flux and spectral radius of every edge of the tile into LDS slots
+// FP32: the limited reconstruction of one cell at each of its three edges,
+// u + limit(grad u . r_k, u, Umax, Umin) per conservative (kernel_Gradient
+// :225-383, kernel_Limiter :456-494 and the reconstruction of
+// kernel_LBJinterpolate :496-549), into the face slots sFc[k][ML].
+// r_k, from the centroid to the midpoint of local edge k (the side opposite
+// vertex k, from P[k+1] to P[k+2]), is (P[k+1] + P[k+2]) / 2 - (P0 + P1 + P2) / 3
+// = (d[k+2] - d[k+1]) / 6 with the side vectors d[i] = P[i+2] - P[i+1].  The
+// cell record holds g[i] = (-d[i].x, d[i].y) * 0.5 / A, so
+// r_k = (g[k+1].x - g[k+2].x, g[k+2].y - g[k+1].y) * A / 3, with A from the
+// stored 1 / A.  Each face slot belongs to one edge: its flux thread reads the
+// state there and then stores its flux in its place (flux_edges).
+__device__ __forceinline__ void face_states(int j, const Vec4<float>& Wi, const CellGeom<float>& cg,
+                                            const int nbk[3], const Vec4<float> nbv[3],
+                                            const FlowT<float>& f, SVec4<float>* sFc, int ML) {
+  float gx[4] = {0.0f, 0.0f, 0.0f, 0.0f}, gy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  float mx[4], mn[4];
+  const float ui[4] = {Wi.d, Wi.du, Wi.dv, Wi.de};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const Vec4<float> c =
+        nbk[k] >= 0 ? nbv[k] : boundary_ghost<float>(-nbk[k], Wi, cg.gx(k), cg.gy(k), f);
+    const float cc[4] = {c.d, c.du, c.dv, c.de};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      mn[q] = k == 0 ? cc[q] : fminf(mn[q], cc[q]);
+      mx[q] = k == 0 ? cc[q] : fmaxf(mx[q], cc[q]);
+      gx[q] += (ui[q] + cc[q]) * cg.gy(k);
+      gy[q] += (ui[q] + cc[q]) * cg.gx(k);
+    }
+  }
+  const float a3 = 1.0f / (3.0f * cg.inv_area);  // A / 3
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    gx[q] *= a3;
+    gy[q] *= a3;
+    mx[q] = fmaxf(ui[q], mx[q]);
+    mn[q] = fminf(ui[q], mn[q]);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+    const float rx = cg.gx(k1) - cg.gx(k2), ry = cg.gy(k2) - cg.gy(k1);
+    float v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = ui[q] + limit(gx[q] * rx + gy[q] * ry, ui[q], mx[q], mn[q]);
+    sFc[k * ML + j] = SVec4<float>{v[0], v[1], v[2], v[3]};
+  }
+}
+
+// LDS arrays of the flux and gather phases (ML: the row stride).
+// FP64: G and B the gradient and bound rows [4][ML] of (d/dx, d/dy) and (max,
+// min) pairs, F and S the flux and spectral radius of each tile edge, and
+// slots the gather table [4][tile cells] of 2 * edge + side.
+// FP32: F the face states of each tile and ring-1 cell [3][ML]
+// (face_states), which the flux phase replaces with the fluxes signed for
+// the face's cell, and S [3][ML] the spectral radius of each face's edge; G,
+// B and slots are unused.
+template <typename R>
+struct FluxLds {
+  const SVec4<R>* W;
+  R* G;
+  R* B;
+  SVec4<R>* F;
+  R* S;
+  unsigned short* slots;
+  int ML;
+};
+
+// Gradient and bounds (FP64) or face states (FP32) of one tile or ring-1 cell.
+template <typename R>
+__device__ __forceinline__ void cell_slopes(int j, const Vec4<R>& Wi, const CellGeom<R>& cg,
+                                            const int nbk[3], const Vec4<R> nbv[3],
+                                            const FlowT<R>& f, const FluxLds<R>& x) {
+  if constexpr (sizeof(R) == 4)
+    face_states(j, Wi, cg, nbk, nbv, f, x.F, x.ML);
+  else
+    gradient_cell(j, Wi, cg, nbk, nbv, f, x.G, x.B, x.ML);
+}
+
+// Roe flux and spectral radius of every edge of the tile into LDS
 // (kernel_Flux2 gpu_solver.cu:651-764 with kernel_LBJinterpolate :496-549,
 // kernel_deltT :385-454).  kPre: e0 is this thread's prefetched first record
-// (otherwise unused).  Each edge also enters its slot, 2 * edge + side, in the
-// gather table sSl[4 * cell + local edge index] of each of its tile cells
-// (cells of the tile have local ids below d.ncell), so the gather reads no
-// slot table from memory (four entries per cell, the fourth unused, so that
-// the gather reads a cell's three with one 8-byte access).
+// (otherwise unused).
+// FP64 reconstructs both sides from the gradient and bound rows and stores
+// the edge's flux and spectral radius at its index; each edge also enters its
+// slot, 2 * edge + side, in the gather table slots[4 * cell + local edge
+// index] of each of its tile cells (cells of the tile have local ids below
+// d.ncell), so the gather reads no slot table from memory (four entries per
+// cell, the fourth unused, so that the gather reads a cell's three with one
+// 8-byte access).
+// FP32 reads the two face states and stores the flux, signed for each side's
+// cell, and the spectral radius in the same two face slots.
 template <typename R, bool kPre = true>
 __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc& d,
-                                           const FlowT<R>& f, const SVec4<R>* sW, const R* sG,
-                                           const R* sB, SVec4<R>* sF, R* sS,
-                                           unsigned short* sSl, int ML, const EdgeRec<R>& e0) {
-  const int s0 = threadIdx.x;
+                                           const FlowT<R>& f, const FluxLds<R>& x,
+                                           const EdgeRec<R>& e0) {
+  constexpr bool kF32 = sizeof(R) == 4;
+  const int s0 = threadIdx.x, ML = x.ML;
   // FP32: the record of this thread's next edge is requested before this
   // edge is computed, so an edge beyond the first blockDim costs no round
   // trip of its own (FP64's 80-byte record would hold 20 more registers)
-  constexpr bool kNext = sizeof(R) == 4;
+  constexpr bool kNext = kF32;
   EdgeRec<R> en{};
   if (kNext && kPre && s0 < d.nedge) en = e0;
   for (int s = threadIdx.x; s < d.nedge; s += blockDim.x) {
@@ -442,36 +503,45 @@ __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc
       e = (kPre && s == s0) ? e0 : a.edges[d.edge0 + s];
     }
     const EdgeView ev = edge_cells(e);
-    if (ev.l < d.ncell) sSl[4 * ev.l + ev.kl] = (unsigned short)(2 * s);
-    if (ev.r >= 0 && ev.r < d.ncell) sSl[4 * ev.r + ev.kr] = (unsigned short)(2 * s + 1);
+    if constexpr (!kF32) {
+      if (ev.l < d.ncell) x.slots[4 * ev.l + ev.kl] = (unsigned short)(2 * s);
+      if (ev.r >= 0 && ev.r < d.ncell) x.slots[4 * ev.r + ev.kr] = (unsigned short)(2 * s + 1);
+    }
+    // FP32 face slots of the two sides (the right one only for r >= 0)
+    const int fl = ev.kl * ML + ev.l, fr = ev.kr * ML + ev.r;
     const R vx = edge_vx(e), vy = edge_vy(e);
     const EdgeNormal<R> en_ = edge_normal(e);
-    const Vec4<R> WL = ld(sW[ev.l]);
+    const Vec4<R> WL = ld(x.W[ev.l]);
     Vec4<R> L = WL, Rr, Rraw;
     if (ev.r >= 0) {
-      Rraw = ld(sW[ev.r]);
+      Rraw = ld(x.W[ev.r]);
       Rr = Rraw;
       if (a.order >= 2) {
-        const int l = ev.l, r = ev.r;
-        const Pair<R>* sG2 = reinterpret_cast<const Pair<R>*>(sG);
-        const Pair<R>* sB2 = reinterpret_cast<const Pair<R>*>(sB);
-        R dl[4], dr[4];
+        if constexpr (kF32) {
+          L = ld(x.F[fl]);
+          Rr = ld(x.F[fr]);
+        } else {
+          const int l = ev.l, r = ev.r;
+          const Pair<R>* sG2 = reinterpret_cast<const Pair<R>*>(x.G);
+          const Pair<R>* sB2 = reinterpret_cast<const Pair<R>*>(x.B);
+          R dl[4], dr[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const Pair<R> gl = sG2[q * ML + l], gr = sG2[q * ML + r];
-          dl[q] = gl.a * e.rlx + gl.b * e.rly;
-          dr[q] = gr.a * e.rrx + gr.b * e.rry;
-        }
-        const R ul[4] = {WL.d, WL.du, WL.dv, WL.de};
-        const R ur[4] = {Rraw.d, Rraw.du, Rraw.dv, Rraw.de};
+          for (int q = 0; q < 4; q++) {
+            const Pair<R> gl = sG2[q * ML + l], gr = sG2[q * ML + r];
+            dl[q] = gl.a * e.rlx + gl.b * e.rly;
+            dr[q] = gr.a * e.rrx + gr.b * e.rry;
+          }
+          const R ul[4] = {WL.d, WL.du, WL.dv, WL.de};
+          const R ur[4] = {Rraw.d, Rraw.du, Rraw.dv, Rraw.de};
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const Pair<R> bl = sB2[q * ML + l], br = sB2[q * ML + r];
-          dl[q] = limit(dl[q], ul[q], bl.a, bl.b);
-          dr[q] = limit(dr[q], ur[q], br.a, br.b);
+          for (int q = 0; q < 4; q++) {
+            const Pair<R> bl = sB2[q * ML + l], br = sB2[q * ML + r];
+            dl[q] = limit(dl[q], ul[q], bl.a, bl.b);
+            dr[q] = limit(dr[q], ur[q], br.a, br.b);
+          }
+          L = Vec4<R>{ul[0] + dl[0], ul[1] + dl[1], ul[2] + dl[2], ul[3] + dl[3]};
+          Rr = Vec4<R>{ur[0] + dr[0], ur[1] + dr[1], ur[2] + dr[2], ur[3] + dr[3]};
         }
-        L = Vec4<R>{ul[0] + dl[0], ul[1] + dl[1], ul[2] + dl[2], ul[3] + dl[3]};
-        Rr = Vec4<R>{ur[0] + dr[0], ur[1] + dr[1], ur[2] + dr[2], ur[3] + dr[3]};
       }
     } else if (ev.r == -1) {
       Rraw = far_ghost(WL, vx, vy, f);
@@ -485,40 +555,61 @@ __device__ __forceinline__ void flux_edges(const TiledArgs<R>& a, const TileDesc
     F.du *= en_.vn;
     F.dv *= en_.vn;
     F.de *= en_.vn;
-    sF[s] = st(F);
-    sS[s] = edge_sr(en_, WL, Rraw, vx, vy, f.gamma);
+    const R sr = edge_sr(en_, WL, Rraw, vx, vy, f.gamma);
+    if constexpr (kF32) {
+      // both slots are this edge's alone, and their states were read above
+      x.F[fl] = st(F);
+      x.S[fl] = sr;
+      if (ev.r >= 0) {
+        x.F[fr] = SVec4<R>{-F.d, -F.du, -F.dv, -F.de};
+        x.S[fr] = sr;
+      }
+    } else {
+      x.F[s] = st(F);
+      x.S[s] = sr;
+    }
   }
 }
 
-// Ordered gather of each tile cell's three edge slots, R and the local time
-// step (kernel_Flux_e2c gpu_solver.cu:768-878).  Returns this thread's
-// minimum of A*CFL/T (global time step mode, which stores no per-cell time
-// step).  The local time step is stored as dt in FP64 and as dt / A in FP32
-// (the RK update's only use of dt is dt R / A, so FP32 folds the division by
-// the area in here and the update reads no area).  ar0: this thread's first
-// cell's (1 / A, A), read at kernel start.  sSl: the slots the flux phase
-// entered (2 * edge + side of each cell's three local edges).
+// Ordered gather of each tile cell's three edges, R and the local time
+// step (kernel_Flux_e2c gpu_solver.cu:768-878): r += F on the edge's left
+// side, r -= F on its right, over the cell's local edges 0, 1, 2.  Returns
+// this thread's minimum of A*CFL/T (global time step mode, which stores no
+// per-cell time step).  The local time step is stored as dt in FP64 and as
+// dt / A in FP32 (the RK update's only use of dt is dt R / A, so FP32 folds
+// the division by the area in here and the update reads no area).  ar0: this
+// thread's first cell's (1 / A, A), read at kernel start.  FP64 finds each
+// flux through the slots the flux phase entered (2 * edge + side); FP32 reads
+// the cell's face slots, which hold the fluxes already signed for the cell.
 template <typename R>
 __device__ __forceinline__ R gather_cells(const TiledArgs<R>& a, const TileDesc& d, const FlowT<R>& f,
-                                          const SVec4<R>* sF, const R* sS, const unsigned short* sSl,
-                                          SVec4<R>* Rout, R* dtout, Pair<R> ar0) {
+                                          const FluxLds<R>& x, SVec4<R>* Rout, R* dtout,
+                                          Pair<R> ar0) {
   const int s0 = threadIdx.x;
   R tmin = R(3.0e38);
   for (int i = threadIdx.x; i < d.ncell; i += blockDim.x) {
     const int g = d.cell0 + i;
-    const uint2 sp = reinterpret_cast<const uint2*>(sSl)[i];
-    const int slk[3] = {(int)(sp.x & 0xffffu), (int)(sp.x >> 16), (int)(sp.y & 0xffffu)};
     R T = 0;
     R r0 = R(0.0), r1 = R(0.0), r2 = R(0.0), r3 = R(0.0);
+    if constexpr (sizeof(R) == 4) {
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const int s = slk[k] >> 1;
-      const SVec4<R> F = sF[s];
-      T += sS[s];
-      // r += F on the edge's left side, r -= F on its right: sg * F is
-      // exact, so r + sg * F (fused or not) rounds as r + F or r - F does
-      const R sg = (slk[k] & 1) ? R(-1.0) : R(1.0);
-      r0 += sg * F.d; r1 += sg * F.du; r2 += sg * F.dv; r3 += sg * F.de;
+      for (int k = 0; k < 3; k++) {
+        const SVec4<R> F = x.F[k * x.ML + i];
+        T += x.S[k * x.ML + i];
+        r0 += F.d; r1 += F.du; r2 += F.dv; r3 += F.de;
+      }
+    } else {
+      const uint2 sp = reinterpret_cast<const uint2*>(x.slots)[i];
+      const int slk[3] = {(int)(sp.x & 0xffffu), (int)(sp.x >> 16), (int)(sp.y & 0xffffu)};
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const int s = slk[k] >> 1;
+        const SVec4<R> F = x.F[s];
+        T += x.S[s];
+        // sg * F is exact, so r + sg * F rounds as r + F or r - F does
+        const R sg = (slk[k] & 1) ? R(-1.0) : R(1.0);
+        r0 += sg * F.d; r1 += sg * F.du; r2 += sg * F.dv; r3 += sg * F.de;
+      }
     }
     Rout[d.st0 + g] = SVec4<R>{r0, r1, r2, r3};
     const Pair<R> ar = (i == s0) ? ar0 : area_pair(a.cg + d.geo0, g);
@@ -563,7 +654,8 @@ __device__ __forceinline__ double block_sum(double acc) {
 
 // --------------------------------------------------------------------------
 // Kernel A: gradient + flux + gather for one tile.
-// LDS: W[nl] | grad[8][nl] | bounds[8][nl] | F[ne] | SR[ne] | gather slots[4][nt] (16-bit)
+// LDS, FP64: W[ML] | grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME] | gather slots[4][nt] (16-bit)
+//      FP32: W[ML] | faces[3][ML] | SR[3][ML]
 // --------------------------------------------------------------------------
 template <typename R>
 __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
@@ -575,13 +667,19 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
   const int nt = d.ncell, nl = d.ncell + d.nhalo;
   const int ML = a.max_local, ME = a.max_edges;
   SVec4<R>* sW = reinterpret_cast<SVec4<R>*>(smem);
-  R* sG = reinterpret_cast<R*>(sW + ML);  // [4][ML] (d/dx, d/dy) pairs
-  R* sB = sG + 8 * ML;                    // [4][ML] (max, min) pairs
-  SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
-  R* sS = reinterpret_cast<R*>(sF + ME);
-  // [4][ncell] gather slots, 8-byte aligned (read as one 8-byte access per cell)
-  unsigned short* sSl = reinterpret_cast<unsigned short*>(
-      smem + ((reinterpret_cast<unsigned char*>(sS + ME) - smem + 7) & ~7));
+  FluxLds<R> x{sW, nullptr, nullptr, nullptr, nullptr, nullptr, ML};
+  if constexpr (sizeof(R) == 4) {
+    x.F = sW + ML;
+    x.S = reinterpret_cast<R*>(x.F + 3 * ML);
+  } else {
+    x.G = reinterpret_cast<R*>(sW + ML);  // [4][ML] (d/dx, d/dy) pairs
+    x.B = x.G + 8 * ML;                   // [4][ML] (max, min) pairs
+    x.F = reinterpret_cast<SVec4<R>*>(x.B + 8 * ML);
+    x.S = reinterpret_cast<R*>(x.F + ME);
+    // [4][ncell] gather slots, 8-byte aligned (read as one 8-byte access per cell)
+    x.slots = reinterpret_cast<unsigned short*>(
+        smem + ((reinterpret_cast<unsigned char*>(x.S + ME) - smem + 7) & ~7));
+  }
   const SVec4<R>* W = a.w_in + d.st0;
 
   // prefetch this thread's first edge record and gather slot; their latency
@@ -602,13 +700,13 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
       Vec4<R> nbv[3];
 #pragma unroll
       for (int k = 0; k < 3; k++) nbv[k] = ld(W[nbk[k] >= 0 ? nbk[k] : g]);
-      gradient_cell<R>(j, Wi, a.cg[d.geo0 + g], nbk, nbv, f, sG, sB, ML);
+      cell_slopes<R>(j, Wi, a.cg[d.geo0 + g], nbk, nbv, f, x);
     }
   }
   __syncthreads();
-  flux_edges<R>(a, d, f, sW, sG, sB, sF, sS, sSl, ML, e0);
+  flux_edges<R>(a, d, f, x, e0);
   __syncthreads();
-  const R tmin = gather_cells<R>(a, d, f, sF, sS, sSl, a.res, a.dt, ar0);
+  const R tmin = gather_cells<R>(a, d, f, x, a.res, a.dt, ar0);
   if (a.global_dt) {
     const R m = block_min(tmin);
     if (threadIdx.x == 0) a.tile_min[t] = m;
@@ -723,9 +821,11 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
 
 // --------------------------------------------------------------------------
 // One kernel per RK stage (see the header comment).
-// LDS: W[MW] | gid[MW4] | union { R[M4 + 1] | R1[MR + 1] ;  grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME] }
-// and the gather slots[4][tile cells] (16-bit) at slot_off: ending where gid ends when that
-// leaves W's entries of tile + ring 1 clear, else after SR
+// LDS: W[MW] | gid[MW4] | union { R[M4 + 1] | R1[MR + 1] ;  flux phase }, the flux phase
+//   FP64: grad[8][ML] | bounds[8][ML] | F[ME] | SR[ME], and the gather slots[4][tile cells]
+//         (16-bit) at slot_off: ending where gid ends when that leaves W's entries of tile +
+//         ring 1 clear, else after SR;
+//   FP32: faces[3][ML] | SR[3][ML]
 // (ML: the slope row stride, S or max_local; MW4: MW rounded up to 4)
 // R holds the previous stage's residual of the tile and rings 1..4, one load
 // per cell, so that smoothing sweep 1 gathers from LDS through the 16-bit
@@ -736,9 +836,9 @@ __global__ void __launch_bounds__(512) update_tile(TiledArgs<R> a) {
 // buffers of steps 4-6 reuse their space (a smaller footprint lets more
 // workgroups share a CU).
 // --------------------------------------------------------------------------
-// S > 0: the slope rows are S apart (S >= max_local), a compile-time stride,
-// so the 16 LDS accesses per cell share one address register with immediate
-// offsets; S == 0: the stride is max_local.
+// S > 0: the slope (FP64) or face (FP32) rows are S apart (S >= max_local), a
+// compile-time stride, so a cell's LDS accesses share one address register
+// with immediate offsets; S == 0: the stride is max_local.
 template <typename R, int S>
 __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -771,15 +871,21 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   int* sGid = reinterpret_cast<int*>(sW + MW);
   SVec4<R>* sR = reinterpret_cast<SVec4<R>*>(sGid + MW4);
   SVec4<R>* sR1 = sR + M4 + 1;
-  R* sG = reinterpret_cast<R*>(sR);
-  R* sB = sG + 8 * ML;
-  SVec4<R>* sF = reinterpret_cast<SVec4<R>*>(sB + 8 * ML);
-  R* sS = reinterpret_cast<R*>(sF + ME);
-  // [4][ncell] gather slots, 8-byte aligned (one 8-byte read per cell): where
-  // the host found room, over the tail of W (rings 2, which only the gradient
-  // reads) and the cell ids (read by sweep 2 and the gradient), both dead in
-  // the flux and gather phases that use the table; else after SR
-  unsigned short* sSl = reinterpret_cast<unsigned short*>(smem + a.slot_off);
+  FluxLds<R> x{sW, nullptr, nullptr, nullptr, nullptr, nullptr, ML};
+  if constexpr (sizeof(R) == 4) {
+    x.F = sR;
+    x.S = reinterpret_cast<R*>(x.F + 3 * ML);
+  } else {
+    x.G = reinterpret_cast<R*>(sR);
+    x.B = x.G + 8 * ML;
+    x.F = reinterpret_cast<SVec4<R>*>(x.B + 8 * ML);
+    x.S = reinterpret_cast<R*>(x.F + ME);
+    // [4][ncell] gather slots, 8-byte aligned (one 8-byte read per cell):
+    // where the host found room, over the tail of W (rings 2, which only the
+    // gradient reads) and the cell ids (read by sweep 2 and the gradient),
+    // both dead in the flux and gather phases that use the table; else after SR
+    x.slots = reinterpret_cast<unsigned short*>(smem + a.slot_off);
+  }
   const int cur = cc.w0_cur ^ a.parity;
   const SVec4<R>* W0 = (cur ? a.w0b : a.w0a) + d.st0;
   SVec4<R>* W0n = (cur ? a.w0a : a.w0b) + d.st0;
@@ -1001,7 +1107,8 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
   if (partial_here) wave_sums(acc, wpart);
   __syncthreads();
   if (partial_here && threadIdx.x == 0) a.partial[t] = sum_of_waves(wpart);
-  // 4. gradient and bounds of the tile and ring 1 from the updated state
+  // 4. gradient and bounds (FP64) or face states (FP32) of the tile and ring 1
+  //    from the updated state
   if (a.order >= 2) {
     for (int j = threadIdx.x; j < NG; j += blockDim.x) {
       int2 lp;
@@ -1015,16 +1122,16 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
       Vec4<R> nbv[3];
 #pragma unroll
       for (int k = 0; k < 3; k++) nbv[k] = ld(sW[nbk[k] >= 0 ? nbk[k] : j]);
-      gradient_cell<R>(j, ld(sW[j]), cgj, nbk, nbv, f, sG, sB, ML);
+      cell_slopes<R>(j, ld(sW[j]), cgj, nbk, nbv, f, x);
     }
   }
   __syncthreads();
   // 5. flux and spectral radius of the tile's edges
-  flux_edges<R, kPreEdge>(a, d, f, sW, sG, sB, sF, sS, sSl, ML, e0);
+  flux_edges<R, kPreEdge>(a, d, f, x, e0);
   __syncthreads();
   // 6. ordered gather: this stage's residual and time step (global time
   //    step: the tile's minimum of A*CFL/T)
-  const R tmin = gather_cells<R>(a, d, f, sF, sS, sSl, a.res, a.dt, ar0);
+  const R tmin = gather_cells<R>(a, d, f, x, a.res, a.dt, ar0);
   if (a.global_dt) {
     const R m = block_min(tmin);
     if (threadIdx.x == 0) a.tile_min[t] = m;
