@@ -297,7 +297,7 @@ class Batch {
   bool ctl_on_host_ = false;         // the host's control blocks are newer than the device's
   hipEvent_t ev_ctl_[2] = {nullptr, nullptr};
   size_t lds_a_ = 0, lds_b_ = 0, lds_s_ = 0;
-  int slot_off_ = 0;  // stage kernel: byte offset of the gather slot table in LDS
+  int slot_off_ = 0;  // FP64 stage kernel: byte offset of the gather slot table in LDS
   bool fused_ = false;      // one kernel per RK stage
   bool r0_valid_ = false;   // (fused) the residual of stage 0 matches the current state
   int max_r4_local_ = 0, max_ring_local_ = 0, max_w_local_ = 0;

@@ -21,14 +21,18 @@
 // step, which needs a grid-wide minimum between flux and update:
 // Kernel A (flux_tile): one workgroup per tile of cells.
 //   1. Green-Gauss gradient and neighbourhood bounds of the tile's cells and
-//      of its first-ring halo, into LDS (kernel_Gradient, gpu_solver.cu:
-//      225-383, with the ghost states of kernel_Boundary :137-223 computed
-//      in place from the boundary cell's own state).
+//      of its first-ring halo (kernel_Gradient, gpu_solver.cu:225-383, with
+//      the ghost states of kernel_Boundary :137-223 computed in place from
+//      the boundary cell's own state): FP64 stores them in LDS rows, FP32
+//      goes on to each cell's limited face states at its three edges
+//      (kernel_Limiter :456-494, kernel_LBJinterpolate :496-549) and stores
+//      those.
 //   2. Roe flux and spectral radius of every edge touching the tile, into
-//      LDS edge slots (kernel_Flux2 :651-764, kernel_deltT :385-454).  Edges
-//      cut by the tile border are evaluated by both tiles from identical
-//      inputs, so both see bitwise the same flux.
-//   3. Ordered gather of each cell's three slots and the local time step
+//      LDS edge slots (FP64) or the two face slots of the edge (FP32)
+//      (kernel_Flux2 :651-764, kernel_deltT :385-454).  Edges cut by the tile
+//      border are evaluated by both tiles from identical inputs, so both see
+//      bitwise the same flux.
+//   3. Ordered gather of each cell's three edges and the local time step
 //      (kernel_Flux_e2c :768-878); writes R and dt.
 // Kernel B (update_tile): one workgroup per tile.
 //   1. First smoothing sweep for the tile and its halo into LDS
@@ -71,9 +75,9 @@ __device__ __forceinline__ Pair<float> area_pair<float>(const CellGeom<float>* c
 }
 
 // Three signed 16-bit values and a fourth small one in 8 bytes (x: bits
-// 0-15 and 16-31, y: bits 0-15, then w in bits 16-31): the gather slots and
-// tile-local neighbour ids of a cell (w = its count of interior neighbours),
-// whose ranges the host checks.  Read every launch, so the narrower record
+// 0-15 and 16-31, y: bits 0-15, then w in bits 16-31): the tile-local
+// neighbour ids of a cell (w = its count of interior neighbours), whose
+// ranges the host checks.  Read every launch, so the narrower record
 // is HBM bandwidth saved.
 __host__ __device__ inline int2 pack_tri16(int a, int b, int c, int w = 0) {
   const unsigned lo = ((unsigned)a & 0xffffu) | (((unsigned)b & 0xffffu) << 16);
@@ -215,7 +219,7 @@ struct TiledArgs {
   int parity;                 // step parity within the run: step-start buffer = w0_cur ^ parity
   int close_prev;             // 1: the workgroup of each case's first tile closes the previous step
   int early_exit;             // 1: test a case's active flag before any load (several cases)
-  int slot_off;               // stage_tile: byte offset of the gather slot table in LDS
+  int slot_off;               // stage_tile, FP64: byte offset of the gather slot table in LDS
   const HistDesc* hist;       // residual history
 };
 
@@ -682,8 +686,8 @@ __global__ void __launch_bounds__(512) flux_tile(TiledArgs<R> a) {
   }
   const SVec4<R>* W = a.w_in + d.st0;
 
-  // prefetch this thread's first edge record and gather slot; their latency
-  // hides behind the gradient phase
+  // prefetch this thread's first edge record and its first cell's 1 / A;
+  // their latency hides behind the gradient phase
   const int s0 = threadIdx.x;
   EdgeRec<R> e0{};
   if (s0 < d.nedge) e0 = a.edges[d.edge0 + s0];
