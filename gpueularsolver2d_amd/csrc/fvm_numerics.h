@@ -24,15 +24,58 @@ struct FlowT {
   T gamma, R, ui, vi, ain, tin, vin, pis, cfl, eps;
 };
 
-// Minimum over the workgroup (blockDim a multiple of 64, at most 1024): a
-// butterfly of wave shuffles, then thread 0 takes the waves' minima.  The
-// minimum of a set does not depend on the order it is taken in (the values
-// are positive time steps; fmin skips a NaN whatever its position), so the
-// result is the same bits as a sequential scan.  Valid in thread 0.
+// The value of v in another lane of the wave, by a DPP lane pattern (kCtrl)
+// applied to the rows in kRowMask; lanes left without a source keep their
+// own value.  DPP moves register values between lanes in the VALU, where a
+// shuffle goes through the LDS crossbar (ds_bpermute).
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ float dpp_from(float v) {
+  const int b = __float_as_int(v);
+  return __int_as_float(__builtin_amdgcn_update_dpp(b, b, kCtrl, kRowMask, 0xf, false));
+}
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ double dpp_from(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = (int)b, hi = (int)(b >> 32);
+  const int l2 = __builtin_amdgcn_update_dpp(lo, lo, kCtrl, kRowMask, 0xf, false);
+  const int h2 = __builtin_amdgcn_update_dpp(hi, hi, kCtrl, kRowMask, 0xf, false);
+  return __longlong_as_double(((long long)h2 << 32) | (unsigned)l2);
+}
+__device__ __forceinline__ float lane63(float v) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double lane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Minimum over the 64 lanes of a wave (all active), returned in every lane:
+// neighbours, then pairs of a quad (quad_perm [1,0,3,2], [2,3,0,1]), the
+// mirrored lane of each half row and of each row of 16 (after which every
+// lane holds its row's minimum), then row 0's into row 1 and row 2's into
+// row 3 (row_bcast:15), and lane 31's into rows 2 and 3 (row_bcast:31), which
+// leaves the wave's minimum in lane 63.
+template <typename R>
+__device__ __forceinline__ R wave_min(R v) {
+  v = fmin(v, dpp_from<0xb1, 0xf>(v));
+  v = fmin(v, dpp_from<0x4e, 0xf>(v));
+  v = fmin(v, dpp_from<0x141, 0xf>(v));
+  v = fmin(v, dpp_from<0x140, 0xf>(v));
+  v = fmin(v, dpp_from<0x142, 0xa>(v));
+  v = fmin(v, dpp_from<0x143, 0xc>(v));
+  return lane63(v);
+}
+
+// Minimum over the workgroup (blockDim a multiple of 64, at most 1024): each
+// wave's minimum, then thread 0 takes the waves' minima.  The minimum of a
+// set does not depend on the order it is taken in (the values are positive
+// time steps; fmin skips a NaN whatever its position), so the result is the
+// same bits as a sequential scan.  Valid in thread 0.
 template <typename R>
 __device__ __forceinline__ R block_min(R v) {
   __shared__ R wmin[16];
-  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  v = wave_min(v);
   if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0)

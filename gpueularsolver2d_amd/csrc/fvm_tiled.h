@@ -1020,12 +1020,12 @@ __global__ void __launch_bounds__(512) stage_tile(TiledArgs<R> a) {
     }
     // Global time step (kernel_TimeStep, gpu_solver.cu:880-936): the
     // previous stage's minimum of A*CFL/T over the case, from its tiles'
-    // minima (requested above, combined here: wave shuffles, then the waves'
-    // minima through LDS after the barrier).  A minimum does not depend on
+    // minima (requested above, combined here: each wave's minimum, then the
+    // waves' minima through LDS after the barrier).  A minimum does not depend on
     // the order it is taken in, so every tile of the case gets the same value.
     if (a.global_dt) {
       R m = fmin(gm0, gm1);
-      for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+      m = wave_min(m);
       if ((threadIdx.x & 63) == 0) s_wmin[threadIdx.x >> 6] = m;
     }
     if (a.smoothing || a.global_dt) __syncthreads();
